@@ -1,0 +1,625 @@
+// lbf_api.cpp -- the C-ABI of include/lbfast_hip.h.
+//
+// A context owns the state of one domain in HBM, laid out as structure of
+// arrays: ten moment fields per ping-pong buffer (P, u, v, w, Pxx, Pyy,
+// Pzz, Pxy, Pxz, Pyz; x fastest), the corrected-velocity ping-pong used as
+// u_old, the phase field ping-pong, the two-component intermediates, and
+// isfluid with a 1-node halo.  All launches go to one HIP stream.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbfast_hip.h"
+#include "lbf_kernels.h"
+
+using namespace lbf;
+
+struct lbf_ctx {
+  lbf_params p{};
+  DevParams dp{};
+  int prec = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  size_t N = 0;
+  size_t tsz = 0;  // bytes of one moment value
+  size_t asz = 0;  // bytes of one aux / phi / force value
+  void* h[2][10] = {};
+  void* U[2][3] = {};
+  void* phi[2] = {};
+  void* aux[7] = {};
+  void* lap = nullptr;
+  void* dvt = nullptr;
+  void* frc[3] = {};
+  int8_t* isfh = nullptr;
+  float* raw[10] = {};
+  void* staging = nullptr;
+  double* partial = nullptr;
+  double* red = nullptr;
+  int nred = 0;
+  int cur = 0, ucur = 0;
+  int step = 0;
+  bool initialized = false;
+  bool isf_set = false;
+  bool raw_pending = false;
+  int ty = 8, tz = 0;
+  std::vector<hipEvent_t> evs;
+  int nev_used = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool timing_valid = false;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+static thread_local std::string g_err;
+
+static int fail(lbf_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf; else g_err = buf;
+  return code;
+}
+#define HIPCHK(c, expr)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail((c), LBF_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+static void* dalloc(lbf_ctx* c, size_t bytes, hipError_t* e) {
+  void* p = nullptr;
+  *e = hipMalloc(&p, bytes);
+  if (*e == hipSuccess) c->allocs.push_back(p);
+  return p;
+}
+
+extern "C" {
+
+int32_t lbf_abi_version(void) { return LBF_ABI_VERSION; }
+
+void lbf_params_default(lbf_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->nx = p->ny = p->nz = 64;
+  p->lattice = 27;
+  p->highorder = 1;
+  p->pbc[0] = p->pbc[1] = p->pbc[2] = 1;
+  p->precision = LBF_PREC_FP32;
+  p->visc1 = p->visc2 = 0.1;
+  p->rho_r = p->rho_b = 1.0;
+  p->tau_diff = 0.1;
+  p->width = 4.0;
+  p->wettab_r = 90.0;
+  lbf_params_derive(p);
+}
+
+// LBFAST.f90:212-219
+int lbf_params_derive(lbf_params* p) {
+  if (!p) return LBF_EINVAL;
+  p->tau1 = p->visc1 * 3.0 + 0.5;
+  p->omega = 1.0 / p->tau1;
+  if (p->twocomponent) {
+    if (p->width == 0.0) return LBF_EINVAL;
+    p->beta = 12.0 * p->sigma / p->width;
+    p->kapp = 1.5 * p->sigma * p->width;
+    p->sharp_c = 4.0 * p->tau_diff / p->width;
+    p->tau2 = p->visc2 * 3.0 + 0.5;
+  }
+  return LBF_OK;
+}
+
+// LBFAST.f90:258-328: walls at non-periodic faces (INTERNAL_OBSTACLES), halo
+// copy (periodic image, else 1), then fluid nodes next to a non-fluid node
+// on the compiled lattice become -1.
+int lbf_classify_isfluid(const lbf_params* p, const int8_t* in, int8_t* out) {
+  if (!p || !out) return LBF_EINVAL;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  if (nx < 1 || ny < 1 || nz < 1) return LBF_EINVAL;
+  auto at = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+  std::vector<int8_t> a((size_t)nx * ny * nz);
+  for (size_t n = 0; n < a.size(); ++n) a[n] = in ? in[n] : 1;
+  if (p->internal_obstacles) {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          if (!p->pbc[0] && (i == 0 || i == nx - 1)) a[at(i, j, k)] = 0;
+          if (!p->pbc[1] && (j == 0 || j == ny - 1)) a[at(i, j, k)] = 0;
+          if (!p->pbc[2] && (k == 0 || k == nz - 1)) a[at(i, j, k)] = 0;
+        }
+  }
+  // value seen through the halo: periodic image, or 1 past a non-periodic face
+  auto look = [&](int i, int j, int k) -> int {
+    const int n3[3] = {nx, ny, nz};
+    int c[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+      if (c[d] < 0 || c[d] >= n3[d]) {
+        if (!p->pbc[d]) return 1;
+        c[d] = (c[d] + n3[d]) % n3[d];
+      }
+    }
+    return a[at(c[0], c[1], c[2])];
+  };
+  static const int EX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, 1, -1};
+  static const int EY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, -1, 1};
+  static const int EZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1};
+  const int nl = p->lattice == 27 ? 27 : 19;
+  std::vector<int8_t> o(a);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        if (a[at(i, j, k)] != 1) continue;
+        for (int l = 1; l < nl; ++l) {
+          const int v = look(i + EX[l], j + EY[l], k + EZ[l]);
+          if (v != 1 && v != -1) { o[at(i, j, k)] = -1; break; }
+        }
+      }
+  memcpy(out, o.data(), o.size());
+  return LBF_OK;
+}
+
+const char* lbf_global_last_error(void) { return g_err.c_str(); }
+const char* lbf_last_error(const lbf_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+static void fill_devparams(lbf_ctx* c) {
+  const lbf_params& p = c->p;
+  DevParams& d = c->dp;
+  d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+  for (int i = 0; i < 3; ++i) d.pbc[i] = p.pbc[i];
+  d.twocomp = p.twocomponent;
+  d.dens = p.twocomponent && p.densratio;
+  d.incomp = d.dens && p.interface_incomp;
+  d.obst = p.internal_obstacles;
+  d.buoy = p.buoyancy_forcing;
+  d.lattice = p.lattice;
+  d.highorder = p.highorder && p.lattice == 27;
+  d.fx = p.fx; d.fy = p.fy; d.fz = p.fz;
+  d.visc1 = p.visc1; d.visc2 = p.visc2;
+  // without DENSRATIO the reference's viscosity mix still reads rho_r/rho_b
+  d.rho_r = p.rho_r; d.rho_b = p.rho_b;
+  d.tau_diff = p.tau_diff; d.sharp_c = p.sharp_c; d.beta = p.beta; d.kapp = p.kapp;
+  d.omega = p.omega;
+  d.wettab = p.wettab_r;
+  // FP16S/BF16S: power-of-two scales that put typical deviations (P ~ 1e-4..1e-2,
+  // u ~ 1e-3..1e-1, Pi - u u ~ 1e-7..1e-3) in the normal range of binary16
+  const double dflt[10] = {16384.0, 1024.0, 1024.0, 1024.0, 1048576.0, 1048576.0, 1048576.0,
+                           1048576.0, 1048576.0, 1048576.0};
+  for (int i = 0; i < 10; ++i) {
+    const double s = p.store_scale[i] > 0 ? p.store_scale[i] : dflt[i];
+    const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
+    d.scale[i] = devc ? (float)s : 1.0f;
+    d.invscale[i] = devc ? (float)(1.0 / s) : 1.0f;
+  }
+}
+
+void lbf_destroy(lbf_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  for (hipEvent_t e : c->evs) hipEventDestroy(e);
+  if (c->t0) hipEventDestroy(c->t0);
+  if (c->t1) hipEventDestroy(c->t1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static void default_tiles(lbf_ctx* c) {
+  c->ty = 8;
+  const long bxy = (long)((c->p.nx + 63) / 64) * ((c->p.ny + c->ty - 1) / c->ty);
+  long nchunks = (2048 + bxy - 1) / bxy;
+  if (nchunks < 1) nchunks = 1;
+  long tz = (c->p.nz + nchunks - 1) / nchunks;
+  if (tz < 8) tz = 8;
+  if (tz > c->p.nz) tz = c->p.nz;
+  c->tz = (int)tz;
+}
+
+int lbf_create(const lbf_params* pin, lbf_ctx** out) {
+  if (!pin || !out) return fail(nullptr, LBF_EINVAL, "null argument");
+  *out = nullptr;
+  lbf_params p = *pin;
+  if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(nullptr, LBF_ESIZE, "domain %dx%dx%d too small", p.nx, p.ny, p.nz);
+  if ((double)p.nx * p.ny * p.nz > 4.0e9) return fail(nullptr, LBF_ESIZE, "domain too large");
+  if (p.lattice != 19 && p.lattice != 27) return fail(nullptr, LBF_EINVAL, "LATTICE %d not supported (19, 27)", p.lattice);
+  if (p.highorder && p.lattice != 27) return fail(nullptr, LBF_EINVAL, "HIGHORDER requires LATTICE 27");
+  if (p.densratio && !p.twocomponent) return fail(nullptr, LBF_EINVAL, "DENSRATIO requires TWOCOMPONENT");
+  if (p.precision < LBF_PREC_FP64 || p.precision > LBF_PREC_BF16S) return fail(nullptr, LBF_EINVAL, "bad precision");
+  if (p.visc1 <= 0 || (p.twocomponent && p.visc2 <= 0)) return fail(nullptr, LBF_EINVAL, "viscosity must be > 0");
+  if (p.twocomponent && p.densratio && (p.rho_r <= 0 || p.rho_b <= 0)) return fail(nullptr, LBF_EINVAL, "rho_r, rho_b must be > 0");
+  if (lbf_params_derive(&p) != LBF_OK) return fail(nullptr, LBF_EINVAL, "width must be non-zero for TWOCOMPONENT");
+
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, LBF_ENODEV, "no HIP device");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(nullptr, LBF_ENODEV, "hipGetDeviceProperties failed");
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, LBF_ENODEV, "device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
+
+  lbf_ctx* c = new lbf_ctx();
+  c->p = p;
+  c->prec = p.precision;
+  c->device = dev;
+  fill_devparams(c);
+  c->N = (size_t)p.nx * p.ny * p.nz;
+  switch (c->prec) {
+    case LBF_PREC_FP64: c->tsz = 8; c->asz = 8; break;
+    case LBF_PREC_FP32: c->tsz = 4; c->asz = 4; break;
+    case LBF_PREC_MIXED: c->tsz = 4; c->asz = 4; break;
+    default: c->tsz = 2; c->asz = 4; break;
+  }
+  hipError_t e = hipSuccess;
+  auto A = [&](size_t bytes) -> void* {
+    if (e != hipSuccess) return nullptr;
+    return dalloc(c, bytes, &e);
+  };
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 10; ++f) c->h[b][f] = A(c->N * c->tsz);
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 3; ++f) c->U[b][f] = A(c->N * c->tsz);
+  if (p.twocomponent) {
+    for (int b = 0; b < 2; ++b) c->phi[b] = A(c->N * c->asz);
+    for (int f = 0; f < 7; ++f) c->aux[f] = A(c->N * c->asz);
+    c->lap = A(c->N * c->asz);
+    c->dvt = A(c->N * c->asz);
+  }
+  for (int f = 0; f < 3; ++f) c->frc[f] = A(c->N * c->asz);
+  c->isfh = (int8_t*)A((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2));
+  c->staging = A(c->N * 8);
+  c->nred = 1024;
+  c->partial = (double*)A((size_t)c->nred * 8 * sizeof(double));
+  c->red = (double*)A(8 * sizeof(double));
+  if (e != hipSuccess) {
+    std::string m = std::string("device allocation failed: ") + hipGetErrorString(e);
+    lbf_destroy(c);
+    return fail(nullptr, LBF_EHIP, "%s", m.c_str());
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->t0) != hipSuccess ||
+      hipEventCreate(&c->t1) != hipSuccess) {
+    lbf_destroy(c);
+    return fail(nullptr, LBF_EHIP, "stream/event creation failed");
+  }
+  // all-fluid isfluid until the host uploads one (halo = 1 everywhere)
+  std::vector<int8_t> ones((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2), 1);
+  if (hipMemcpy(c->isfh, ones.data(), ones.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    lbf_destroy(c);
+    return fail(nullptr, LBF_EHIP, "isfluid init failed");
+  }
+  default_tiles(c);
+  *out = c;
+  return LBF_OK;
+}
+
+int lbf_set_tile(lbf_ctx* c, int32_t ty, int32_t tz) {
+  if (!c) return LBF_EINVAL;
+  if (ty == 0 && tz == 0) { default_tiles(c); return LBF_OK; }
+  if (ty != 0 && ty != 8 && ty != 13) return fail(c, LBF_EINVAL, "tile_y must be 8 or 13");
+  if (ty) c->ty = ty;
+  if (tz) {
+    if (tz < 1) return fail(c, LBF_EINVAL, "tile_z must be >= 1");
+    c->tz = tz > c->p.nz ? c->p.nz : tz;
+  }
+  return LBF_OK;
+}
+
+int32_t lbf_get_step(const lbf_ctx* c) { return c ? c->step : -1; }
+
+// storage kind of a field id
+static int field_kind(const lbf_ctx* c, int id, bool* moment) {
+  *moment = (id >= 1 && id <= 10) || (id >= 27 && id <= 36);
+  const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
+  if (*moment || (id >= 12 && id <= 14)) {
+    if (c->prec == LBF_PREC_FP64) return K_F64;
+    if (c->prec == LBF_PREC_FP16S) return K_F16;
+    if (c->prec == LBF_PREC_BF16S) return K_BF16;
+    return K_F32;
+  }
+  (void)devc;
+  return c->asz == 8 ? K_F64 : K_F32;
+}
+
+static void* field_ptr(lbf_ctx* c, int id) {
+  if (id >= 1 && id <= 10) return c->h[c->cur][id - 1];
+  if (id == LBF_FIELD_PHI) return c->phi[c->cur];
+  if (id >= 12 && id <= 14) return c->U[c->ucur][id - 12];
+  if (id >= 15 && id <= 17) return c->frc[id - 15];
+  if (id >= 18 && id <= 24) return c->aux[id - 18];
+  if (id == LBF_FIELD_LAPPHI) return c->lap;
+  if (id == LBF_FIELD_DIVTHN) return c->dvt;
+  if (id >= 27 && id <= 36) return c->h[1 - c->cur][id - 27];
+  if (id == LBF_FIELD_PHI_O) return c->phi[1 - c->cur];
+  return nullptr;
+}
+
+void* lbf_field_device_ptr(lbf_ctx* c, int id) { return c ? field_ptr(c, id) : nullptr; }
+
+int lbf_upload_field(lbf_ctx* c, int id, const void* host, int dtype) {
+  if (!c || !host) return LBF_EINVAL;
+  if (dtype != 0 && dtype != 1) return fail(c, LBF_EINVAL, "dtype must be 0 (float64) or 1 (float32)");
+  void* dst = field_ptr(c, id);
+  if (!dst) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
+  bool mom;
+  const int kind = field_kind(c, id, &mom);
+  const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
+  const size_t bytes = c->N * (dtype == 0 ? 8 : 4);
+  HIPCHK(c, hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
+  if (mom && devc) {
+    if (id > 10) return fail(c, LBF_EINVAL, "O-buffer upload not supported with 16-bit storage");
+    // staged in FP32; encoded as deviations by lbf_init_state
+    if (!c->raw[id - 1]) {
+      hipError_t e;
+      c->raw[id - 1] = (float*)dalloc(c, c->N * 4, &e);
+      if (e != hipSuccess) return fail(c, LBF_EHIP, "raw staging alloc failed");
+    }
+    HIPCHK(c, launch_convert_in(c->staging, dtype, c->raw[id - 1], K_F32, c->N, 1.0f, c->stream));
+    c->raw_pending = true;
+  } else {
+    float sc = 1.0f;
+    if (devc && id >= 12 && id <= 14) sc = c->dp.scale[id - 11];
+    HIPCHK(c, launch_convert_in(c->staging, dtype, dst, kind, c->N, sc, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+int lbf_download_field(lbf_ctx* c, int id, void* host, int dtype) {
+  if (!c || !host) return LBF_EINVAL;
+  if (dtype != 0 && dtype != 1) return fail(c, LBF_EINVAL, "dtype must be 0 (float64) or 1 (float32)");
+  const size_t bytes = c->N * (dtype == 0 ? 8 : 4);
+  if (id == LBF_FIELD_RHOPHI) {
+    // observable of copy_print (print_module.f90:734-760)
+    std::vector<double> ph(c->N);
+    if (c->p.twocomponent) {
+      int rc = lbf_download_field(c, LBF_FIELD_PHI, ph.data(), 0);
+      if (rc) return rc;
+      for (size_t n = 0; n < c->N; ++n)
+        if (c->p.densratio) ph[n] = c->p.rho_r * ph[n] + (1.0 - ph[n]) * c->p.rho_b;
+    } else {
+      int rc = lbf_download_field(c, LBF_FIELD_P, ph.data(), 0);
+      if (rc) return rc;
+    }
+    if (dtype == 0) memcpy(host, ph.data(), bytes);
+    else for (size_t n = 0; n < c->N; ++n) ((float*)host)[n] = (float)ph[n];
+    return LBF_OK;
+  }
+  void* src = field_ptr(c, id);
+  if (!src) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
+  bool mom;
+  const int kind = field_kind(c, id, &mom);
+  const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
+  if (mom) {
+    if (devc && c->raw_pending) return fail(c, LBF_ESTATE, "moments uploaded but not encoded: call lbf_init_state");
+    void* const* buf = (id <= 10) ? c->h[c->cur] : c->h[1 - c->cur];
+    const int comp = (id <= 10) ? id - 1 : id - 27;
+    HIPCHK(c, launch_decode_field((const void* const*)buf, comp, c->staging, dtype, c->dp, c->prec, c->stream));
+  } else {
+    float inv = 1.0f;
+    if (devc && id >= 12 && id <= 14) inv = c->dp.invscale[id - 11];
+    HIPCHK(c, launch_convert_out(src, kind, c->staging, dtype, c->N, inv, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+int lbf_upload_isfluid(lbf_ctx* c, const int8_t* host) {
+  if (!c || !host) return LBF_EINVAL;
+  const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+  bool any_solid = false;
+  for (size_t n = 0; n < c->N; ++n) {
+    if (host[n] != 0 && host[n] != 1 && host[n] != -1) return fail(c, LBF_EINVAL, "isfluid values must be -1, 0 or 1");
+    any_solid |= host[n] == 0;
+  }
+  if (any_solid && !c->p.internal_obstacles)
+    return fail(c, LBF_EINVAL, "solid nodes require INTERNAL_OBSTACLES (internal_obstacles = 1)");
+  const int hx = nx + 2, hy = ny + 2, hz = nz + 2;
+  std::vector<int8_t> h((size_t)hx * hy * hz, 1);
+  for (int k = 0; k < hz; ++k)
+    for (int j = 0; j < hy; ++j)
+      for (int i = 0; i < hx; ++i) {
+        int c3[3] = {i - 1, j - 1, k - 1};
+        const int n3[3] = {nx, ny, nz};
+        bool one = false;
+        for (int d = 0; d < 3; ++d) {
+          if (c3[d] < 0 || c3[d] >= n3[d]) {
+            if (!c->p.pbc[d]) one = true;
+            c3[d] = (c3[d] + n3[d]) % n3[d];
+          }
+        }
+        h[(size_t)i + (size_t)hx * ((size_t)j + (size_t)hy * k)] =
+            one ? 1 : host[(size_t)c3[0] + (size_t)nx * ((size_t)c3[1] + (size_t)ny * c3[2])];
+      }
+  HIPCHK(c, hipMemcpy(c->isfh, h.data(), h.size(), hipMemcpyHostToDevice));
+  c->isf_set = true;
+  return LBF_OK;
+}
+
+static StageArgs stage_args(lbf_ctx* c) {
+  StageArgs a{};
+  for (int f = 0; f < 10; ++f) { a.S[f] = c->h[c->cur][f]; a.O[f] = c->h[1 - c->cur][f]; }
+  for (int f = 0; f < 3; ++f) { a.uin[f] = c->U[c->ucur][f]; a.uout[f] = c->U[1 - c->ucur][f]; a.frc[f] = c->frc[f]; }
+  a.phiS = c->phi[c->cur];
+  a.phiO = c->phi[1 - c->cur];
+  for (int f = 0; f < 7; ++f) a.aux[f] = c->aux[f];
+  a.lap = c->lap;
+  a.dvt = c->dvt;
+  a.isfh = c->isfh;
+  a.dp = c->dp;
+  return a;
+}
+
+static FusedArgs fused_args(lbf_ctx* c, bool inl) {
+  FusedArgs a{};
+  for (int f = 0; f < 10; ++f) { a.min[f] = c->h[c->cur][f]; a.mout[f] = c->h[1 - c->cur][f]; }
+  for (int f = 0; f < 3; ++f) { a.uin[f] = c->U[c->ucur][f]; a.uout[f] = c->U[1 - c->ucur][f]; a.frc[f] = c->frc[f]; }
+  a.phin = c->phi[c->cur];
+  a.phiout = c->phi[1 - c->cur];
+  for (int f = 0; f < 4; ++f) a.aux[f] = c->aux[f];
+  a.lap = c->lap;
+  a.dvt = c->dvt;
+  a.isfh = c->isfh;
+  a.dp = c->dp;
+  a.ty = c->ty;
+  a.tz = c->tz;
+  (void)inl;
+  return a;
+}
+
+int lbf_init_state(lbf_ctx* c) {
+  if (!c) return LBF_EINVAL;
+  const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
+  if (devc) {
+    for (int f = 0; f < 10; ++f)
+      if (!c->raw[f]) return fail(c, LBF_ESTATE, "moment field %d not uploaded", f + 1);
+    HIPCHK(c, launch_encode_moments(c->h[c->cur], c->raw, c->dp, c->prec, c->stream));
+    c->raw_pending = false;
+  }
+  // O := S (init_conditions_module.f90:584-587), u_old := u
+  for (int f = 0; f < 10; ++f) HIPCHK(c, launch_copy(c->h[1 - c->cur][f], c->h[c->cur][f], c->N * c->tsz, c->stream));
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 3; ++f) HIPCHK(c, launch_copy(c->U[b][f], c->h[c->cur][1 + f], c->N * c->tsz, c->stream));
+  if (c->p.twocomponent) {
+    HIPCHK(c, launch_copy(c->phi[1 - c->cur], c->phi[c->cur], c->N * c->asz, c->stream));
+    for (int f = 0; f < 7; ++f) HIPCHK(c, launch_fill_zero(c->aux[f], c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->lap, c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->dvt, c->N * c->asz, c->stream));
+  }
+  for (int f = 0; f < 3; ++f) HIPCHK(c, launch_fill_zero(c->frc[f], c->N * c->asz, c->stream));
+  // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0
+  if (c->p.twocomponent && c->p.internal_obstacles) {
+    StageArgs a = stage_args(c);
+    HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->step = 0;
+  c->initialized = true;
+  return LBF_OK;
+}
+
+static int run_fused(lbf_ctx* c, bool inl) {
+  FusedArgs a = fused_args(c, inl);
+  if (c->nev_used + 2 > (int)c->evs.size()) {
+    for (int i = 0; i < 2; ++i) {
+      hipEvent_t ev;
+      HIPCHK(c, hipEventCreate(&ev));
+      c->evs.push_back(ev);
+    }
+  }
+  HIPCHK(c, hipEventRecord(c->evs[c->nev_used], c->stream));
+  HIPCHK(c, launch_fused(a, c->prec, inl, c->stream));
+  HIPCHK(c, hipEventRecord(c->evs[c->nev_used + 1], c->stream));
+  c->nev_used += 2;
+  return LBF_OK;
+}
+
+int lbf_step(lbf_ctx* c, int32_t nsteps) {
+  if (!c) return LBF_EINVAL;
+  if (!c->initialized) return fail(c, LBF_ESTATE, "lbf_init_state must be called before lbf_step");
+  if (nsteps < 0) return fail(c, LBF_EINVAL, "nsteps < 0");
+  c->nev_used = 0;
+  HIPCHK(c, hipEventRecord(c->t0, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    ++c->step;
+    if (c->p.twocomponent) {
+      StageArgs a = stage_args(c);
+      HIPCHK(c, launch_stage(LBF_STAGE_NORM, a, c->prec, c->stream));
+      if (c->p.internal_obstacles) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+      HIPCHK(c, launch_stage(LBF_STAGE_DIV_THETA_N, a, c->prec, c->stream));
+    }
+    int rc = run_fused(c, true);
+    if (rc) return rc;
+    c->cur ^= 1;
+    c->ucur ^= 1;
+  }
+  HIPCHK(c, hipEventRecord(c->t1, c->stream));
+  c->timing_valid = nsteps > 0;
+  return LBF_OK;
+}
+
+int lbf_last_step_timing(lbf_ctx* c, double* total_ms, double* fused_ms) {
+  if (!c || !total_ms || !fused_ms) return LBF_EINVAL;
+  if (!c->timing_valid) return fail(c, LBF_ESTATE, "no timed lbf_step call");
+  HIPCHK(c, hipEventSynchronize(c->t1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+  *total_ms = ms;
+  double f = 0;
+  for (int i = 0; i + 1 < c->nev_used; i += 2) {
+    float m2 = 0;
+    HIPCHK(c, hipEventElapsedTime(&m2, c->evs[i], c->evs[i + 1]));
+    f += m2;
+  }
+  *fused_ms = f;
+  return LBF_OK;
+}
+
+int lbf_begin_step(lbf_ctx* c) {
+  if (!c) return LBF_EINVAL;
+  if (!c->initialized) return fail(c, LBF_ESTATE, "lbf_init_state must be called first");
+  ++c->step;
+  return LBF_OK;
+}
+
+int lbf_stage(lbf_ctx* c, int id) {
+  if (!c) return LBF_EINVAL;
+  if (!c->initialized) return fail(c, LBF_ESTATE, "lbf_init_state must be called first");
+  const bool two = c->p.twocomponent != 0, obst = c->p.internal_obstacles != 0;
+  switch (id) {
+    case LBF_STAGE_NORM:
+    case LBF_STAGE_DIV_THETA_N:
+    case LBF_STAGE_UPDATE_PHI:
+      if (!two) return LBF_OK;
+      break;
+    case LBF_STAGE_PHI_INT_BOUNDARY:
+      if (!two || !obst) return LBF_OK;
+      break;
+    case LBF_STAGE_LB_INT_BOUNDARY:
+      if (!obst) return LBF_OK;
+      break;
+    case LBF_STAGE_MOMENTS:
+      break;
+    case LBF_STAGE_FUSED: {
+      if (c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S)
+        return fail(c, LBF_EINVAL, "the stage-path fused kernel supports FP64/FP32/MIXED storage");
+      c->nev_used = 0;
+      int rc = run_fused(c, false);
+      if (rc) return rc;
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      return LBF_OK;
+    }
+    default:
+      return fail(c, LBF_EINVAL, "unknown stage %d", id);
+  }
+  StageArgs a = stage_args(c);
+  HIPCHK(c, launch_stage(id, a, c->prec, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+int lbf_end_step(lbf_ctx* c) {
+  if (!c) return LBF_EINVAL;
+  c->cur ^= 1;
+  c->ucur ^= 1;
+  return LBF_OK;
+}
+
+int lbf_reduce_fp64(lbf_ctx* c, double out[8]) {
+  if (!c || !out) return LBF_EINVAL;
+  const void* S[10];
+  for (int f = 0; f < 10; ++f) S[f] = c->h[c->cur][f];
+  HIPCHK(c, launch_reduce(S, c->phi[c->cur], c->isfh, c->dp, c->prec, c->partial, c->nred, c->red, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+int lbf_synchronize(lbf_ctx* c) {
+  if (!c) return LBF_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,466 @@
+// lbf_fused.hip -- fused stream+collide from moments for gfx950.
+//
+// Replaces fused_LB_kernel (lb_cuda_fused_module.f90:426-750) and, in the
+// "inline" variant used by lbf_step, also the moments stage
+// (lb_cuda_moments_module.f90:19-341), the phase-field update
+// (lb_cuda_update_phi_module.f90:19-104) and the near-wall regularised
+// bounce-back (lb_cuda_boundary_module.f90:19-216), so that one launch reads
+// the state once and writes the new state once.
+//
+// Arithmetic.  The reference rebuilds 27 post-collision populations as
+//   f*_l = feq_l + (1-omega) fneq_l + F_l/2
+// with feq/fneq the second-order terms plus, under HIGHORDER, Minv_mrt(k,l)
+// times the monomials E_k and polynomials N_k (k = 10..26, fused:641-702).
+// Every term is a Hermite coefficient times Minv_mrt(k,l) = w_l H_k(c_l)/b_k,
+// and for D3Q27 the basis is the tensor product of the 1-D basis
+// {1, c, c^2-1/3} (vars_module.f90:522-664), so
+//   f*_{pqr} = sum_{abc} g_a(p) g_b(q) g_c(r) A_{abc},
+//   A_0 = P, A_i = u_i + a_i/2, A_ij = u_i u_j + (1-omega) Pn_ij + (u_i a_j + u_j a_i)/2,
+//   A_k = E_k + (1-omega) N_k (k >= 10)
+// (a = force/rho_phi; Guo forcing of fused:705-706 is exactly the order-1/2
+// Hermite part).  The 27 populations then cost three 1-D passes (135 ops)
+// instead of the dense 27x27 sum.  D3Q19 is not a tensor product; its 19
+// populations are built directly from the order <= 2 coefficients.
+//
+// Streaming + moments.  out_m(x) = sum_l M_ml f*_l(x - c_l) separates by
+// axis too.  A block owns a 64-wide x segment (one wave per y row, lanes =
+// x) of TY output rows plus one halo row on each side, and marches z.  For
+// each source plane it:
+//   x: shifts the p = +-1 populations one lane (ds_bpermute) and contracts
+//      over p with (1, c, c^2-1/3); the two x-halo nodes of every row are
+//      computed by one extra wave and handed over through LDS;
+//   y: exchanges the q = +-1 partial sums with the neighbouring rows through
+//      LDS and contracts over q;
+//   z: adds the r = -1/0/+1 partial sums into three per-lane accumulator
+//      planes; the lowest is complete and is stored.
+// Every node of the block's (x,y) footprint is collided once per plane; the
+// recomputation is only the two y-halo rows, the x-halo wave and the two
+// z-halo planes of the march.
+//
+// Periodicity: indices wrap in-kernel (the reference's intpbc halo copies,
+// mpi_module.f90:2820-2918).  isfluid is read from a copy with the
+// reference's 1-node halo (periodic image or 1, LBFAST.f90:258-328).
+#include "lbf_common.h"
+#include "lbf_kernels.h"
+#include "lbf_physics.h"
+
+namespace lbf {
+
+// ---------------------------------------------------------------------------
+// Post-collision populations f[cidx(p,q,r)] of one node.
+template <int LAT, bool HO, class R>
+__device__ __forceinline__ void collide(const NodeIn<R>& n, R* f) {
+  const R om1 = (R)1 - n.om;
+  const R u = n.u, v = n.v, w = n.w;
+  // order <= 2 Hermite coefficients (post-collision, with half forcing)
+  const R A000 = n.P;
+  const R A100 = u + (R)0.5 * n.ax, A010 = v + (R)0.5 * n.ay, A001 = w + (R)0.5 * n.az;
+  const R A200 = u * u + om1 * n.xx + u * n.ax;
+  const R A020 = v * v + om1 * n.yy + v * n.ay;
+  const R A002 = w * w + om1 * n.zz + w * n.az;
+  const R A110 = u * v + om1 * n.xy + (R)0.5 * (u * n.ay + v * n.ax);
+  const R A101 = u * w + om1 * n.xz + (R)0.5 * (u * n.az + w * n.ax);
+  const R A011 = v * w + om1 * n.yz + (R)0.5 * (v * n.az + w * n.ay);
+  if constexpr (LAT == 27) {
+    R A[27];  // A[a + 3b + 9c]
+#define AI(a, b, c) A[(a) + 3 * (b) + 9 * (c)]
+    AI(0, 0, 0) = A000;
+    AI(1, 0, 0) = A100; AI(0, 1, 0) = A010; AI(0, 0, 1) = A001;
+    AI(2, 0, 0) = A200; AI(0, 2, 0) = A020; AI(0, 0, 2) = A002;
+    AI(1, 1, 0) = A110; AI(1, 0, 1) = A101; AI(0, 1, 1) = A011;
+    if constexpr (HO) {
+      const R pxx = n.xx, pyy = n.yy, pzz = n.zz, pxy = n.xy, pxz = n.xz, pyz = n.yz;
+      const R uu = u * u, vv = v * v, ww = w * w;
+      // E_k and N_k of fused:668-702 (k = 10..26), N_k transcribed as written
+      AI(2, 1, 0) = uu * v + om1 * ((R)2 * pxy * u + pxx * v);
+      AI(2, 0, 1) = uu * w + om1 * ((R)2 * pxz * u + pxx * w);
+      AI(1, 2, 0) = u * vv + om1 * (pyy * u + (R)2 * pxy * v);
+      AI(1, 0, 2) = u * ww + om1 * (pzz * u + (R)2 * pxz * w);
+      AI(0, 1, 2) = v * ww + om1 * (pzz * v + (R)2 * pyz * w);
+      AI(0, 2, 1) = vv * w + om1 * ((R)2 * pyz * v + pyy * w);
+      AI(1, 1, 1) = u * v * w + om1 * (pyz * u + pxz * v + pxy * w);
+      AI(2, 2, 0) = uu * vv + om1 * (pyy * uu + (R)2 * v * ((R)3 * pxy * u + pxx * v));
+      AI(2, 0, 2) = uu * ww + om1 * (pzz * uu + (R)2 * w * ((R)3 * pxz * u + pxx * w));
+      AI(0, 2, 2) = vv * ww + om1 * (pzz * vv + (R)2 * w * ((R)3 * pyz * v + pyy * w));
+      AI(1, 1, 2) = u * v * ww + om1 * (pzz * u * v + w * ((R)3 * pyz * u + (R)3 * pxz * v + (R)2 * pxy * w));
+      AI(1, 2, 1) = u * vv * w + om1 * ((R)2 * pyz * u * v + pxz * vv + (R)2 * pyy * u * w + (R)4 * pxy * v * w);
+      AI(2, 1, 1) = uu * v * w + om1 * (pyz * uu + (R)2 * pxz * u * v + (R)4 * pxy * u * w + (R)2 * pxx * v * w);
+      AI(2, 1, 2) = uu * v * ww +
+                    om1 * (pzz * uu * v + (R)3 * w * (pyz * uu + (R)2 * pxz * u * v + (R)2 * pxy * u * w + pxx * v * w));
+      AI(2, 2, 1) = uu * vv * w +
+                    om1 * ((R)2 * u * v * (pyz * u + pxz * v) + (R)2 * pyy * uu * w + v * ((R)10 * pxy * u + (R)3 * pxx * v) * w);
+      AI(1, 2, 2) = u * vv * ww +
+                    om1 * (pzz * u * vv + (R)3 * w * ((R)2 * pyz * u * v + pxz * vv + pyy * u * w + (R)2 * pxy * v * w));
+      AI(2, 2, 2) = uu * vv * ww +
+                    om1 * (pzz * uu * vv + w * ((R)6 * u * v * (pyz * u + pxz * v) + (R)3 * pyy * uu * w +
+                                                (R)2 * v * ((R)7 * pxy * u + (R)2 * pxx * v) * w));
+    } else {
+      AI(2, 1, 0) = 0; AI(2, 0, 1) = 0; AI(1, 2, 0) = 0; AI(1, 0, 2) = 0; AI(0, 1, 2) = 0; AI(0, 2, 1) = 0;
+      AI(1, 1, 1) = 0; AI(2, 2, 0) = 0; AI(2, 0, 2) = 0; AI(0, 2, 2) = 0; AI(1, 1, 2) = 0; AI(1, 2, 1) = 0;
+      AI(2, 1, 1) = 0; AI(2, 1, 2) = 0; AI(2, 2, 1) = 0; AI(1, 2, 2) = 0; AI(2, 2, 2) = 0;
+    }
+#undef AI
+    // x pass: A[a][b][c] -> B[p][b][c]
+    R B[27];
+#pragma unroll
+    for (int bc = 0; bc < 9; ++bc) recon3(A[3 * bc + 0], A[3 * bc + 1], A[3 * bc + 2], B[3 * bc + 0], B[3 * bc + 1], B[3 * bc + 2]);
+    // y pass: B[p][b][c] -> A[p][q][c]
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int pp = 0; pp < 3; ++pp)
+        recon3(B[pp + 0 + 9 * c], B[pp + 3 + 9 * c], B[pp + 6 + 9 * c], A[pp + 0 + 9 * c], A[pp + 3 + 9 * c], A[pp + 6 + 9 * c]);
+    // z pass: A[p][q][c] -> f[p][q][r]
+#pragma unroll
+    for (int pq = 0; pq < 9; ++pq) recon3(A[pq], A[pq + 9], A[pq + 18], f[pq], f[pq + 9], f[pq + 18]);
+  } else {
+    // D3Q19: f_l = w_l [A0 + 3 c.A1 + 4.5 sum_ab (c_a c_b - delta/3) A2_ab]
+#pragma unroll
+    for (int r = -1; r <= 1; ++r)
+#pragma unroll
+      for (int q = -1; q <= 1; ++q)
+#pragma unroll
+        for (int p = -1; p <= 1; ++p) {
+          const int c = cidx(p, q, r);
+          if (!present<LAT>(p, q, r)) { f[c] = 0; continue; }
+          const R wl = (R)weight<LAT>(p, q, r);
+          R s = A000;
+          if (p) s += (R)(3 * p) * A100;
+          if (q) s += (R)(3 * q) * A010;
+          if (r) s += (R)(3 * r) * A001;
+          s += (R)(4.5 * (p * p - 1.0 / 3.0)) * A200;
+          s += (R)(4.5 * (q * q - 1.0 / 3.0)) * A020;
+          s += (R)(4.5 * (r * r - 1.0 / 3.0)) * A002;
+          if (p * q) s += (R)(9 * p * q) * A110;
+          if (p * r) s += (R)(9 * p * r) * A101;
+          if (q * r) s += (R)(9 * q * r) * A011;
+          f[c] = wl * s;
+        }
+  }
+}
+
+template <class R>
+__device__ __forceinline__ R shfl_from_lower(R v) {  // lane L receives lane L-1
+  return __shfl_up(v, 1, 64);
+}
+template <class R>
+__device__ __forceinline__ R shfl_from_upper(R v) {  // lane L receives lane L+1
+  return __shfl_down(v, 1, 64);
+}
+
+// ---------------------------------------------------------------------------
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
+__global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
+  typedef Codec<PREC> C;
+  typedef typename C::R R;
+  typedef typename C::T T;
+  typedef typename Aux<PREC>::T TA;
+  constexpr int NW = TY + 2;  // row waves (TY output rows + 2 halo rows)
+
+  __shared__ R sUP[NW][9][64];
+  __shared__ R sDN[NW][9][64];
+  __shared__ R sHX[NW][2][9];
+
+  const DevParams& dp = a.dp;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int x0 = blockIdx.x * 64;
+  const int XW = min(64, nx - x0);
+  const int y0 = blockIdx.y * TY;
+  const int z0 = blockIdx.z * a.tz;
+  const int zend = min(z0 + a.tz, nz);
+  const bool halo_wave = (wave == NW);
+
+  int w, xs, side = 0;
+  bool active = true;
+  if (!halo_wave) {
+    w = wave;
+    xs = x0 + lane;
+  } else {
+    w = lane >> 1;
+    side = lane & 1;
+    active = lane < 2 * NW;
+    if (!active) w = 0;
+    xs = side ? x0 + XW : x0 - 1;
+  }
+  const int ys = y0 - 1 + w;
+  const bool out_row = !halo_wave && w >= 1 && w <= TY && ys < ny;
+  const bool out_node = out_row && lane < XW;
+  const int xw = ((xs % nx) + nx) % nx;  // lanes past a short row wrap more than once
+  const int yw = ((ys % ny) + ny) % ny;  // ys may exceed 2ny when ny < TY
+
+  T* const* Sin = (T* const*)a.min;
+  T* const* Sout = (T* const*)a.mout;
+
+  R lo[10], mid[10], hi[10];
+#pragma unroll
+  for (int m = 0; m < 10; ++m) { lo[m] = 0; mid[m] = 0; hi[m] = 0; }
+
+  const R third = (R)(1.0 / 3.0), twothird = (R)(2.0 / 3.0);
+
+  for (int zsrc = z0 - 1; zsrc <= zend; ++zsrc) {
+    const int zw = wrapg(zsrc, nz);
+    const size_t idx = nidx(xw, yw, zw, nx, ny);
+    const int8_t isf = isf_at(a.isfh, xs, ys, zsrc, nx, ny, nz);
+    const bool own_plane = zsrc >= z0 && zsrc < zend;
+
+    // ---- 1. load + (inline) moments stage + collision -----------------------
+    R f[27];
+    R lbc[10];
+    bool has_lbc = false;
+    {
+      NodeIn<R> in;
+      Mom<R> m = load_mom<PREC>(Sin, idx, dp);
+      R phl = 0, rinv = (R)1, Fx, Fy, Fz;
+      const bool solid = dp.obst && isf == 0;
+      if constexpr (TWO) phl = (R)((const TA*)a.phin)[idx];
+      if constexpr (!INLINE) {
+        if (TWO && dp.dens) rinv = (R)1.0 / ((R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b);
+        Fx = (R)((const TA*)a.frc[0])[idx] * rinv;
+        Fy = (R)((const TA*)a.frc[1])[idx] * rinv;
+        Fz = (R)((const TA*)a.frc[2])[idx] * rinv;
+      } else {
+        R gfx = 0, gfy = 0, gfz = 0, lap = 0, dvt = 0, uox = 0, uoy = 0, uoz = 0;
+        if constexpr (TWO) {
+          const R mg = (R)((const TA*)a.aux[3])[idx];
+          gfx = (R)((const TA*)a.aux[0])[idx] * mg;
+          gfy = (R)((const TA*)a.aux[1])[idx] * mg;
+          gfz = (R)((const TA*)a.aux[2])[idx] * mg;
+          lap = (R)((const TA*)a.lap)[idx];
+          dvt = (R)((const TA*)a.dvt)[idx];
+          if (dp.dens) {
+            uox = C::ld((const T*)a.uin[0], idx, dp.invscale[1]);
+            uoy = C::ld((const T*)a.uin[1], idx, dp.invscale[2]);
+            uoz = C::ld((const T*)a.uin[2], idx, dp.invscale[3]);
+          }
+        }
+        R tFx, tFy, tFz;
+        moments_stage<R>(dp, TWO, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+        // the reference stores S and the forces at storage precision between
+        // the moments and fused kernels
+        m.u = C::round(m.u); m.v = C::round(m.v); m.w = C::round(m.w);
+        m.xx = C::round(m.xx); m.yy = C::round(m.yy); m.zz = C::round(m.zz);
+        m.xy = C::round(m.xy); m.xz = C::round(m.xz); m.yz = C::round(m.yz);
+        tFx = C::round(tFx); tFy = C::round(tFy); tFz = C::round(tFz);
+        if (TWO && dp.dens) rinv = (R)1.0 / ((R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b);
+        Fx = tFx * rinv; Fy = tFy * rinv; Fz = tFz * rinv;
+        if (out_node && own_plane && !solid) {
+          if constexpr (TWO) {
+            if (dp.dens) {
+              C::stv((T*)a.uout[0], idx, m.u, dp.scale[1]);
+              C::stv((T*)a.uout[1], idx, m.v, dp.scale[2]);
+              C::stv((T*)a.uout[2], idx, m.w, dp.scale[3]);
+            }
+            // update_phi (update_phi:88-93), u after the moments stage
+            const R t = -(R)dp.sharp_c * dvt;
+            const R po = phl - m.u * (R)0.5 * (gfx) - m.v * (R)0.5 * (gfy) - m.w * (R)0.5 * (gfz) +
+                         (R)dp.tau_diff * lap + t;
+            ((TA*)a.phiout)[idx] = (TA)po;
+          }
+          if (dp.obst && isf == -1) {
+            // LB_int_boundary (boundary:117-212) on the post-moments state
+            has_lbc = true;
+            const R cssq = (R)(1.0 / 3.0);
+            R om;
+            if constexpr (TWO) {
+              const R rph = dp.dens ? (R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b : (R)1;
+              const R visc = ((R)dp.rho_r * (R)dp.visc1 * phl + ((R)1 - phl) * (R)dp.visc2 * (R)dp.rho_b) / rph;
+              om = (R)1.0 / (visc * (R)3 + (R)0.5);
+            } else {
+              om = (R)dp.omega;
+            }
+            const R pxx = m.xx - m.u * m.u, pyy = m.yy - m.v * m.v, pzz = m.zz - m.w * m.w;
+            const R pxy = m.xy - m.u * m.v, pxz = m.xz - m.u * m.w, pyz = m.yz - m.v * m.w;
+#pragma unroll
+            for (int q = 0; q < 10; ++q) lbc[q] = 0;
+#pragma unroll
+            for (int c = 0; c < 27; ++c) {
+              const int p = c % 3 - 1, q = (c / 3) % 3 - 1, r = c / 9 - 1;
+              if (c == 13 || !present<LAT>(p, q, r)) continue;
+              if (isf_at(a.isfh, xs - p, ys - q, zsrc - r, nx, ny, nz) != 0) continue;
+              const R wl = (R)weight<LAT>(p, q, r);
+              const R fneq = wl * (R)4.5 *
+                             (((R)(p * p) - cssq) * pxx + ((R)(q * q) - cssq) * pyy + ((R)(r * r) - cssq) * pzz +
+                              (R)(2 * p * q) * pxy + (R)(2 * p * r) * pxz + (R)(2 * q * r) * pyz);
+              const R fp = wl * m.P + ((R)1 - om) * fneq;
+              lbc[0] += fp;
+              lbc[1] += fp * (R)p; lbc[2] += fp * (R)q; lbc[3] += fp * (R)r;
+              lbc[4] += fp * ((R)(p * p) - cssq); lbc[5] += fp * ((R)(q * q) - cssq); lbc[6] += fp * ((R)(r * r) - cssq);
+              lbc[7] += fp * (R)(p * q); lbc[8] += fp * (R)(p * r); lbc[9] += fp * (R)(q * r);
+            }
+          }
+        }
+      }
+      if (solid) {
+        m.P = m.u = m.v = m.w = m.xx = m.yy = m.zz = m.xy = m.xz = m.yz = 0;
+        Fx = Fy = Fz = 0;
+      }
+      in.P = m.P; in.u = m.u; in.v = m.v; in.w = m.w;
+      in.xx = m.xx - m.u * m.u; in.yy = m.yy - m.v * m.v; in.zz = m.zz - m.w * m.w;
+      in.xy = m.xy - m.u * m.v; in.xz = m.xz - m.u * m.w; in.yz = m.yz - m.v * m.w;
+      in.ax = Fx; in.ay = Fy; in.az = Fz;
+      if constexpr (TWO) {
+        const R visc = ((R)dp.rho_r * (R)dp.visc1 * phl + ((R)1 - phl) * (R)dp.visc2 * (R)dp.rho_b) * rinv;
+        in.om = (R)1.0 / (visc * (R)3 + (R)0.5);
+      } else {
+        in.om = (R)dp.omega;
+      }
+      collide<LAT, HO, R>(in, f);
+    }
+
+    // ---- 2. x-halo slices through LDS -------------------------------------------
+    if (halo_wave && active) {
+      const int p = side ? -1 : 1;  // left halo streams +x into lane 0, right -x into lane XW-1
+#pragma unroll
+      for (int r = -1; r <= 1; ++r)
+#pragma unroll
+        for (int q = -1; q <= 1; ++q) sHX[w][side][(q + 1) + 3 * (r + 1)] = f[cidx(p, q, r)];
+    }
+    __syncthreads();
+
+    R G0[9];  // q = 0 partial sums kept in registers: [d][r]
+    if (!halo_wave) {
+      // ---- 3. x: shift by one lane, contract over p ---------------------------
+      R G[3][3][3];  // [d][q][r]
+#pragma unroll
+      for (int r = -1; r <= 1; ++r)
+#pragma unroll
+        for (int q = -1; q <= 1; ++q) {
+          const int qr = (q + 1) + 3 * (r + 1);
+          R fm_in = 0, fp_in = 0;
+          if (present<LAT>(1, q, r)) {
+            fp_in = shfl_from_lower(f[cidx(1, q, r)]);
+            const R hl = sHX[w][0][qr];
+            if (lane == 0) fp_in = hl;
+          }
+          if (present<LAT>(-1, q, r)) {
+            fm_in = shfl_from_upper(f[cidx(-1, q, r)]);
+            const R hr = sHX[w][1][qr];
+            if (lane == XW - 1) fm_in = hr;
+          }
+          const R f0 = present<LAT>(0, q, r) ? f[cidx(0, q, r)] : (R)0;
+          const R s = fm_in + f0 + fp_in;
+          G[0][q + 1][r + 1] = s;
+          G[1][q + 1][r + 1] = fp_in - fm_in;
+          G[2][q + 1][r + 1] = twothird * (fp_in + fm_in) - third * f0;
+        }
+      // ---- 4a. y: hand the q = +-1 partial sums to the neighbouring rows ------
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          if (w < NW - 1) sUP[w][d * 3 + r][lane] = G[d][2][r];
+          if (w > 0) sDN[w][d * 3 + r][lane] = G[d][0][r];
+          G0[d * 3 + r] = G[d][1][r];
+        }
+    }
+    __syncthreads();
+
+    if (out_row) {
+      // ---- 4b. y: contract over q ------------------------------------------------
+      // H[d][e][r] for (d,e) in {00, 01, 02, 10, 11, 20}
+      R H00[3], H01[3], H02[3], H10[3], H11[3], H20[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        {
+          const R gp = sUP[w - 1][0 * 3 + r][lane], gm = sDN[w + 1][0 * 3 + r][lane], g0 = G0[0 * 3 + r];
+          H00[r] = gm + g0 + gp;
+          H01[r] = gp - gm;
+          H02[r] = twothird * (gp + gm) - third * g0;
+        }
+        {
+          const R gp = sUP[w - 1][1 * 3 + r][lane], gm = sDN[w + 1][1 * 3 + r][lane], g0 = G0[1 * 3 + r];
+          H10[r] = gm + g0 + gp;
+          H11[r] = gp - gm;
+        }
+        {
+          const R gp = sUP[w - 1][2 * 3 + r][lane], gm = sDN[w + 1][2 * 3 + r][lane], g0 = G0[2 * 3 + r];
+          H20[r] = gm + g0 + gp;
+        }
+      }
+      // ---- 5. z: r = +1 -> plane zsrc+1 (hi), 0 -> zsrc (mid), -1 -> zsrc-1 (lo)
+      // outputs: 0 P(000) 1 u(100) 2 v(010) 3 w(001) 4 xx(200) 5 yy(020) 6 zz(002)
+      //          7 xy(110) 8 xz(101) 9 yz(011); index [0]=r-1 [1]=r0 [2]=r+1
+      // (d,e) = (0,0): f = 0, 1, 2
+      hi[0] += H00[2]; mid[0] += H00[1]; lo[0] += H00[0];
+      hi[3] += H00[2]; lo[3] -= H00[0];
+      hi[6] += twothird * H00[2]; mid[6] -= third * H00[1]; lo[6] += twothird * H00[0];
+      // (0,1): f = 0 -> v, f = 1 -> yz
+      hi[2] += H01[2]; mid[2] += H01[1]; lo[2] += H01[0];
+      hi[9] += H01[2]; lo[9] -= H01[0];
+      // (1,0): f = 0 -> u, f = 1 -> xz
+      hi[1] += H10[2]; mid[1] += H10[1]; lo[1] += H10[0];
+      hi[8] += H10[2]; lo[8] -= H10[0];
+      // (0,2) -> yy, (2,0) -> xx, (1,1) -> xy
+      hi[5] += H02[2]; mid[5] += H02[1]; lo[5] += H02[0];
+      hi[4] += H20[2]; mid[4] += H20[1]; lo[4] += H20[0];
+      hi[7] += H11[2]; mid[7] += H11[1]; lo[7] += H11[0];
+      if (has_lbc) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) mid[q] += lbc[q];
+      }
+      // ---- 6. plane zsrc-1 is complete --------------------------------------------
+      const int zo = zsrc - 1;
+      if (lane < XW && zo >= z0 && zo < zend) {
+        Mom<R> o;
+        o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
+        o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
+        store_mom<PREC>(Sout, nidx(xw, yw, zo, nx, ny), o, dp);
+      }
+#pragma unroll
+      for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
+static hipError_t launch_t(const FusedArgs& a, hipStream_t s) {
+  dim3 grid((a.dp.nx + 63) / 64, (a.dp.ny + TY - 1) / TY, (a.dp.nz + a.tz - 1) / a.tz);
+  dim3 block((TY + 3) * 64);
+  hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <int PREC, bool TWO, bool INLINE, int TY>
+static hipError_t launch_lat(const FusedArgs& a, hipStream_t s) {
+  if (a.dp.lattice == 19) return launch_t<19, false, PREC, TWO, INLINE, TY>(a, s);
+  if (a.dp.highorder) return launch_t<27, true, PREC, TWO, INLINE, TY>(a, s);
+  return launch_t<27, false, PREC, TWO, INLINE, TY>(a, s);
+}
+
+template <int PREC, bool INLINE, int TY>
+static hipError_t launch_two(const FusedArgs& a, hipStream_t s) {
+  if (a.dp.twocomp) return launch_lat<PREC, true, INLINE, TY>(a, s);
+  return launch_lat<PREC, false, INLINE, TY>(a, s);
+}
+
+template <int PREC, bool INLINE>
+static hipError_t launch_ty(const FusedArgs& a, hipStream_t s) {
+  switch (a.ty) {
+    case 8: return launch_two<PREC, INLINE, 8>(a, s);
+    case 13: return launch_two<PREC, INLINE, 13>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s) {
+  if (inline_moments) {
+    switch (prec) {
+      case PREC_FP64: return launch_ty<PREC_FP64, true>(a, s);
+      case PREC_FP32: return launch_ty<PREC_FP32, true>(a, s);
+      case PREC_MIXED: return launch_ty<PREC_MIXED, true>(a, s);
+      case PREC_FP16S: return launch_ty<PREC_FP16S, true>(a, s);
+      case PREC_BF16S: return launch_ty<PREC_BF16S, true>(a, s);
+    }
+  } else {
+    switch (prec) {
+      case PREC_FP64: return launch_ty<PREC_FP64, false>(a, s);
+      case PREC_FP32: return launch_ty<PREC_FP32, false>(a, s);
+      case PREC_MIXED: return launch_ty<PREC_MIXED, false>(a, s);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace lbf

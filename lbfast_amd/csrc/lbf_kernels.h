@@ -1,0 +1,58 @@
+// lbf_kernels.h -- launch interfaces between the C-ABI layer and the kernels.
+#pragma once
+#include "lbf_common.h"
+
+namespace lbf {
+
+struct FusedArgs {
+  const void* min[10];  // S moments (storage codec)
+  void* mout[10];       // O moments
+  const void* uin[3];   // previous corrected velocity (2c DENSRATIO)
+  void* uout[3];        // this step's corrected velocity
+  const void* phin;     // phase field S
+  void* phiout;         // phase field O
+  const void* aux[4];   // n_x n_y n_z |grad phi| (two-component, inline)
+  const void* lap;      // laplacian of phi
+  const void* dvt;      // div(phi(1-phi) n)
+  const void* frc[3];   // forces (stage path)
+  const int8_t* isfh;   // isfluid with 1-node halo
+  DevParams dp;
+  int ty, tz;           // tile rows, z-chunk length
+};
+
+// Stage-path buffers (the reference's per-kernel arguments).
+struct StageArgs {
+  void* S[10];
+  void* O[10];
+  const void* uin[3];
+  void* uout[3];
+  void* phiS;
+  void* phiO;
+  void* aux[7];
+  void* lap;
+  void* dvt;
+  void* frc[3];
+  const int8_t* isfh;
+  DevParams dp;
+};
+
+hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s);
+hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s);
+
+// storage conversion: host-order double/float array -> field (codec-aware)
+hipError_t launch_encode_moments(void* const* S, const float* const* raw, const DevParams& dp, int prec, hipStream_t s);
+hipError_t launch_decode_field(const void* const* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
+                               hipStream_t s);
+hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, size_t n, float scale,
+                             hipStream_t s);
+hipError_t launch_convert_out(const void* src, int src_kind, void* dst, int dst_dtype, size_t n, float invscale,
+                              hipStream_t s);
+hipError_t launch_reduce(const void* const* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
+                         double* partial, int nblocks, double* out8, hipStream_t s);
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// element kinds for convert: 0 double, 1 float, 2 half (scaled), 3 bf16 (scaled)
+enum Kind { K_F64 = 0, K_F32 = 1, K_F16 = 2, K_BF16 = 3 };
+
+}  // namespace lbf
