@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the LB time step on one MI355X.
+
+Workload (the metric's configuration): two-component D3Q27 HIGHORDER
+DENSRATIO, 512^3 periodic, a Laplace droplet (R = 128, w = 3, density ratio
+100, sigma = 0.01), FP32 moments in SoA layout.  A "step" is one full time
+step of lbf_step (phase-field stencils + the fused stream/collide/moments/
+phase-update kernel) over the whole domain; state stays resident in HBM.
+
+Prints ONE JSON line:
+  value           MLUPS = nodes * steps / t / 1e6 (solid nodes counted, LBFAST.f90:500)
+  roofline        the fused kernel against HBM: algorithmic bytes per launch
+                  (112 B/node FP32: 10 moments + 3 u_old + phi, read and written)
+                  over its average launch time from HIP events on the context stream
+  fp16s / bf16s   the same workload with moments stored as scaled 16-bit
+                  deviations, FP32 arithmetic; speed-up and velocity error vs FP64
+  cpu_baseline    the oracle (a CPU port of the reference algorithm) timed
+                  on this host on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from lbfast_amd import setups  # noqa: E402
+from lbfast_amd.params import make_params, PREC_FP32, PREC_FP64, PREC_FP16S, PREC_BF16S  # noqa: E402
+
+METRIC = "MLUPS & % HBM peak, D3Q27 2-comp 512^3; FP16-store vs FP32 at stated error"
+HBM_PEAK_GBS = 8000.0
+A100_2C_D3Q27H_SP_MLUPS = 800.0  # README.md:405, 1x A100, 2c 512^3 D3Q27h SP
+BYTES_PER_LUP = {PREC_FP32: 112.0, PREC_FP64: 224.0, PREC_FP16S: 60.0, PREC_BF16S: 60.0}
+
+
+def droplet_params(n, precision):
+    return make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True,
+                       visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=0.01, tau_diff=0.1, sigma=0.01, width=3.0,
+                       precision=precision)
+
+
+def droplet_fields(n, dtype=np.float32):
+    R = n[0] / 4.0
+    f = setups.droplet(n, R, 3.0, [n[0] / 2 + 0.5, n[1] / 2 + 0.5, n[2] / 2 + 0.5], 0.01, 1.0, 0.01,
+                       noise=1e-6, seed=20261202)
+    return {k: v.astype(dtype) for k, v in f.items()}
+
+
+def load_domain(p, fields):
+    from lbfast_amd import hip
+    d = hip.Domain(p)
+    dt = np.float32 if next(iter(fields.values())).dtype == np.float32 else np.float64
+    from lbfast_amd.params import HFIELDS, HFIELD_NAMES, FIELD_PHI
+    for fid, nm in zip(HFIELDS, HFIELD_NAMES):
+        d.upload(fid, fields[nm], dtype=dt)
+    d.upload(FIELD_PHI, fields["phi"], dtype=dt)
+    d.init_state()
+    return d
+
+
+def time_domain(d, steps, warmup):
+    d.step(warmup)
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.step(steps)
+    d.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms, fused_ms = d.timing()
+    return wall, total_ms, fused_ms
+
+
+def precision_errors(n=(96, 96, 96), steps=400):
+    """Velocity / density / phase error of FP32, FP16S, BF16S storage against
+    the FP64 GPU run (itself held to the FP64 oracle by the parity tests)."""
+    f64 = droplet_fields(n, np.float64)
+    out = {}
+    ref = load_domain(droplet_params(n, PREC_FP64), f64)
+    ref.step(steps)
+    ru = np.stack([ref.download(i) for i in (2, 3, 4)])
+    rphi = ref.download(11)
+    rP = ref.download(1)
+    ref.close()
+    for name, prec in (("fp32", PREC_FP32), ("fp16s", PREC_FP16S), ("bf16s", PREC_BF16S)):
+        d = load_domain(droplet_params(n, prec), f64)
+        d.step(steps)
+        u = np.stack([d.download(i) for i in (2, 3, 4)])
+        phi = d.download(11)
+        P = d.download(1)
+        d.close()
+        out[name] = {
+            "vel_rel_l2": float(np.linalg.norm(u - ru) / np.linalg.norm(ru)),
+            "vel_linf": float(np.abs(u - ru).max()),
+            "P_rel_l2": float(np.linalg.norm(P - rP) / np.linalg.norm(rP)),
+            "phi_rel_l2": float(np.linalg.norm(phi - rphi) / np.linalg.norm(rphi)),
+        }
+    return {"grid": list(n), "steps": steps, "vs": "fp64 GPU run", "fields": out}
+
+
+def cpu_baseline(target_s=15.0):
+    """The oracle on this host: same physics (2c D3Q27h DENSRATIO droplet,
+    FP32) on a 64^3 box, steps chosen to take ~target_s seconds."""
+    import oracle
+    n = (64, 64, 64)
+    p = droplet_params(n, PREC_FP32)
+    f = droplet_fields(n, np.float64)
+    o = oracle.Oracle(p)
+    o.load(f)
+    o.time_steps(1)  # first touch of the arrays
+    dt1 = o.time_steps(2) / 2
+    steps = max(2, min(400, int(target_s / max(dt1, 1e-3))))
+    t = o.time_steps(steps)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    mlups = n[0] * n[1] * n[2] * steps / t / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++/OpenMP CPU port of the reference stages), FP32, 2c D3Q27h densratio "
+                      f"droplet 64^3, {steps} steps in {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage legs")
+    ap.add_argument("--only", default="", help="run one leg: fp32 | fp16s | bf16s | 1c")
+    ap.add_argument("--tile", default="", help="ty,tz override of the fused kernel tile")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU sharding is implemented); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    n = (args.n, args.n, args.n)
+    nodes = n[0] * n[1] * n[2]
+    tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+    print(f"[bench] building {n} droplet fields", file=sys.stderr, flush=True)
+    fields = droplet_fields(n, np.float32)
+
+    def leg(prec):
+        d = load_domain(droplet_params(n, prec), fields)
+        if tile:
+            d.set_tile(*tile)
+        wall, total_ms, fused_ms = time_domain(d, args.steps, args.warmup)
+        d.close()
+        mlups = nodes * args.steps / wall / 1e6
+        fused_avg_ms = fused_ms / args.steps
+        achieved = BYTES_PER_LUP[prec] * nodes / (fused_avg_ms * 1e-3) / 1e9
+        print(f"[bench] prec={prec} {mlups:.1f} MLUPS wall {wall*1e3/args.steps:.3f} ms/step "
+              f"events {total_ms/args.steps:.3f} ms/step fused {fused_avg_ms:.3f} ms", file=sys.stderr, flush=True)
+        return {"mlups": mlups, "ms_per_step": wall * 1e3 / args.steps, "fused_ms": fused_avg_ms,
+                "achieved_gbs": achieved, "step_gbs": BYTES_PER_LUP[prec] * mlups * 1e6 / 1e9}
+
+    res = {}
+    legs = [args.only] if args.only else ["fp32", "fp16s", "bf16s"]
+    if args.no_precision and not args.only:
+        legs = ["fp32"]
+    precs = {"fp32": PREC_FP32, "fp16s": PREC_FP16S, "bf16s": PREC_BF16S}
+    for name in legs:
+        if name in precs:
+            res[name] = leg(precs[name])
+    main_leg = res.get("fp32") or next(iter(res.values()))
+    del fields
+
+    out = {
+        "metric": METRIC,
+        "value": round(main_leg["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_leg["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(main_leg["mlups"] / A100_2C_D3Q27H_SP_MLUPS, 2),
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"laplace_droplet_2c_d3q27h_densratio_{args.n}^3", "lattice": "D3Q27 HIGHORDER",
+                   "components": 2, "densratio": 100, "nodes": nodes, "storage": "fp32 SoA",
+                   "baseline": "1x A100 2c 512^3 D3Q27h SP 800 MLUPS (README.md:405)"},
+        "roofline": {"bound": "hbm", "achieved": round(main_leg["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_leg["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "fused_kernel (inline moments)",
+                     "bytes_per_node": BYTES_PER_LUP[PREC_FP32], "kernel_ms": round(main_leg["fused_ms"], 4)},
+        "hbm_frac_step": round(main_leg["step_gbs"] / HBM_PEAK_GBS, 4),
+    }
+    for name in ("fp16s", "bf16s"):
+        if name in res and "fp32" in res:
+            out[name] = {"value": round(res[name]["mlups"], 1), "speedup_vs_fp32": round(res[name]["mlups"] / res["fp32"]["mlups"], 3),
+                         "kernel_ms": round(res[name]["fused_ms"], 4),
+                         "kernel_gbs_alg": round(res[name]["achieved_gbs"], 1)}
+    if not args.no_precision and not args.only:
+        print("[bench] precision errors vs FP64", file=sys.stderr, flush=True)
+        out["precision_error"] = precision_errors()
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

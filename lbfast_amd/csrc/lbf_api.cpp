@@ -27,15 +27,17 @@ struct lbf_ctx {
   size_t N = 0;
   size_t tsz = 0;  // bytes of one moment value
   size_t asz = 0;  // bytes of one aux / phi / force value
-  void* h[2][10] = {};
-  void* U[2][3] = {};
+  void* h[2] = {};    // 10-field sets (S/O ping-pong)
+  void* U[2] = {};    // 3-field sets: corrected velocity (u_old ping-pong)
   void* phi[2] = {};
-  void* aux[7] = {};
+  void* aux = nullptr;  // 7-field set (stage path)
   void* lap = nullptr;
   void* dvt = nullptr;
-  void* frc[3] = {};
+  void* frc = nullptr;  // 3-field set (stage path)
+  void* pre = nullptr;  // 5-field set (fast path phase-field prep)
   int8_t* isfh = nullptr;
-  float* raw[10] = {};
+  float* raw = nullptr;  // 10-field FP32 staging of 16-bit storage uploads
+  bool raw_have[10] = {};
   void* staging = nullptr;
   double* partial = nullptr;
   double* red = nullptr;
@@ -45,7 +47,7 @@ struct lbf_ctx {
   bool initialized = false;
   bool isf_set = false;
   bool raw_pending = false;
-  int ty = 8, tz = 0;
+  int ty = 8, tz = 0, tz_prep = 64;
   std::vector<hipEvent_t> evs;
   int nev_used = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -177,13 +179,18 @@ static void fill_devparams(lbf_ctx* c) {
   d.buoy = p.buoyancy_forcing;
   d.lattice = p.lattice;
   d.highorder = p.highorder && p.lattice == 27;
-  d.fx = p.fx; d.fy = p.fy; d.fz = p.fz;
-  d.visc1 = p.visc1; d.visc2 = p.visc2;
+  d.nn = (size_t)p.nx * p.ny * p.nz;
   // without DENSRATIO the reference's viscosity mix still reads rho_r/rho_b
-  d.rho_r = p.rho_r; d.rho_b = p.rho_b;
-  d.tau_diff = p.tau_diff; d.sharp_c = p.sharp_c; d.beta = p.beta; d.kapp = p.kapp;
-  d.omega = p.omega;
-  d.wettab = p.wettab_r;
+  Phys<double>& q = d.pd;
+  q.fx = p.fx; q.fy = p.fy; q.fz = p.fz;
+  q.visc1 = p.visc1; q.visc2 = p.visc2; q.rho_r = p.rho_r; q.rho_b = p.rho_b;
+  q.tau_diff = p.tau_diff; q.sharp_c = p.sharp_c; q.beta = p.beta; q.kapp = p.kapp;
+  q.omega = p.omega; q.wettab = p.wettab_r;
+  Phys<float>& f = d.pf;
+  f.fx = (float)q.fx; f.fy = (float)q.fy; f.fz = (float)q.fz;
+  f.visc1 = (float)q.visc1; f.visc2 = (float)q.visc2; f.rho_r = (float)q.rho_r; f.rho_b = (float)q.rho_b;
+  f.tau_diff = (float)q.tau_diff; f.sharp_c = (float)q.sharp_c; f.beta = (float)q.beta; f.kapp = (float)q.kapp;
+  f.omega = (float)q.omega; f.wettab = (float)q.wettab;
   // FP16S/BF16S: power-of-two scales that put typical deviations (P ~ 1e-4..1e-2,
   // u ~ 1e-3..1e-1, Pi - u u ~ 1e-7..1e-3) in the normal range of binary16
   const double dflt[10] = {16384.0, 1024.0, 1024.0, 1024.0, 1048576.0, 1048576.0, 1048576.0,
@@ -208,14 +215,25 @@ void lbf_destroy(lbf_ctx* c) {
 }
 
 static void default_tiles(lbf_ctx* c) {
-  c->ty = 8;
+  // rows per block: 8 or 13, whichever needs fewer waves for the y extent
+  // (a block of ty rows runs ty + 3 waves: two halo rows and the x-halo wave)
+  const long w8 = (long)((c->p.ny + 7) / 8) * 11, w13 = (long)((c->p.ny + 12) / 13) * 16;
+  c->ty = w13 < w8 ? 13 : 8;
+  // z-chunks so the grid holds ~1024 blocks (4 per CU, one resident at a time)
   const long bxy = (long)((c->p.nx + 63) / 64) * ((c->p.ny + c->ty - 1) / c->ty);
-  long nchunks = (2048 + bxy - 1) / bxy;
+  long nchunks = (1024 + bxy - 1) / bxy;
   if (nchunks < 1) nchunks = 1;
   long tz = (c->p.nz + nchunks - 1) / nchunks;
   if (tz < 8) tz = 8;
   if (tz > c->p.nz) tz = c->p.nz;
   c->tz = (int)tz;
+  const long pxy = (long)((c->p.nx + 63) / 64) * ((c->p.ny + 7) / 8);
+  long pch = (2048 + pxy - 1) / pxy;
+  if (pch < 1) pch = 1;
+  long tzp = (c->p.nz + pch - 1) / pch;
+  if (tzp < 8) tzp = 8;
+  if (tzp > c->p.nz) tzp = c->p.nz;
+  c->tz_prep = (int)tzp;
 }
 
 int lbf_create(const lbf_params* pin, lbf_ctx** out) {
@@ -256,17 +274,16 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     if (e != hipSuccess) return nullptr;
     return dalloc(c, bytes, &e);
   };
-  for (int b = 0; b < 2; ++b)
-    for (int f = 0; f < 10; ++f) c->h[b][f] = A(c->N * c->tsz);
-  for (int b = 0; b < 2; ++b)
-    for (int f = 0; f < 3; ++f) c->U[b][f] = A(c->N * c->tsz);
+  for (int b = 0; b < 2; ++b) c->h[b] = A(10 * c->N * c->tsz);
+  for (int b = 0; b < 2; ++b) c->U[b] = A(3 * c->N * c->tsz);
   if (p.twocomponent) {
     for (int b = 0; b < 2; ++b) c->phi[b] = A(c->N * c->asz);
-    for (int f = 0; f < 7; ++f) c->aux[f] = A(c->N * c->asz);
+    c->aux = A(7 * c->N * c->asz);
     c->lap = A(c->N * c->asz);
     c->dvt = A(c->N * c->asz);
+    c->pre = A(5 * c->N * c->asz);
   }
-  for (int f = 0; f < 3; ++f) c->frc[f] = A(c->N * c->asz);
+  c->frc = A(3 * c->N * c->asz);
   c->isfh = (int8_t*)A((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2));
   c->staging = A(c->N * 8);
   c->nred = 1024;
@@ -321,15 +338,18 @@ static int field_kind(const lbf_ctx* c, int id, bool* moment) {
   return c->asz == 8 ? K_F64 : K_F32;
 }
 
+static void* fset(void* base, int f, size_t n, size_t sz) { return base ? (char*)base + (size_t)f * n * sz : nullptr; }
+
 static void* field_ptr(lbf_ctx* c, int id) {
-  if (id >= 1 && id <= 10) return c->h[c->cur][id - 1];
+  const size_t N = c->N, T = c->tsz, A = c->asz;
+  if (id >= 1 && id <= 10) return fset(c->h[c->cur], id - 1, N, T);
   if (id == LBF_FIELD_PHI) return c->phi[c->cur];
-  if (id >= 12 && id <= 14) return c->U[c->ucur][id - 12];
-  if (id >= 15 && id <= 17) return c->frc[id - 15];
-  if (id >= 18 && id <= 24) return c->aux[id - 18];
+  if (id >= 12 && id <= 14) return fset(c->U[c->ucur], id - 12, N, T);
+  if (id >= 15 && id <= 17) return fset(c->frc, id - 15, N, A);
+  if (id >= 18 && id <= 24) return fset(c->aux, id - 18, N, A);
   if (id == LBF_FIELD_LAPPHI) return c->lap;
   if (id == LBF_FIELD_DIVTHN) return c->dvt;
-  if (id >= 27 && id <= 36) return c->h[1 - c->cur][id - 27];
+  if (id >= 27 && id <= 36) return fset(c->h[1 - c->cur], id - 27, N, T);
   if (id == LBF_FIELD_PHI_O) return c->phi[1 - c->cur];
   return nullptr;
 }
@@ -349,12 +369,13 @@ int lbf_upload_field(lbf_ctx* c, int id, const void* host, int dtype) {
   if (mom && devc) {
     if (id > 10) return fail(c, LBF_EINVAL, "O-buffer upload not supported with 16-bit storage");
     // staged in FP32; encoded as deviations by lbf_init_state
-    if (!c->raw[id - 1]) {
+    if (!c->raw) {
       hipError_t e;
-      c->raw[id - 1] = (float*)dalloc(c, c->N * 4, &e);
+      c->raw = (float*)dalloc(c, 10 * c->N * 4, &e);
       if (e != hipSuccess) return fail(c, LBF_EHIP, "raw staging alloc failed");
     }
-    HIPCHK(c, launch_convert_in(c->staging, dtype, c->raw[id - 1], K_F32, c->N, 1.0f, c->stream));
+    HIPCHK(c, launch_convert_in(c->staging, dtype, c->raw + (size_t)(id - 1) * c->N, K_F32, c->N, 1.0f, c->stream));
+    c->raw_have[id - 1] = true;
     c->raw_pending = true;
   } else {
     float sc = 1.0f;
@@ -392,9 +413,9 @@ int lbf_download_field(lbf_ctx* c, int id, void* host, int dtype) {
   const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
   if (mom) {
     if (devc && c->raw_pending) return fail(c, LBF_ESTATE, "moments uploaded but not encoded: call lbf_init_state");
-    void* const* buf = (id <= 10) ? c->h[c->cur] : c->h[1 - c->cur];
+    void* buf = (id <= 10) ? c->h[c->cur] : c->h[1 - c->cur];
     const int comp = (id <= 10) ? id - 1 : id - 27;
-    HIPCHK(c, launch_decode_field((const void* const*)buf, comp, c->staging, dtype, c->dp, c->prec, c->stream));
+    HIPCHK(c, launch_decode_field(buf, comp, c->staging, dtype, c->dp, c->prec, c->stream));
   } else {
     float inv = 1.0f;
     if (devc && id >= 12 && id <= 14) inv = c->dp.invscale[id - 11];
@@ -439,32 +460,37 @@ int lbf_upload_isfluid(lbf_ctx* c, const int8_t* host) {
 
 static StageArgs stage_args(lbf_ctx* c) {
   StageArgs a{};
-  for (int f = 0; f < 10; ++f) { a.S[f] = c->h[c->cur][f]; a.O[f] = c->h[1 - c->cur][f]; }
-  for (int f = 0; f < 3; ++f) { a.uin[f] = c->U[c->ucur][f]; a.uout[f] = c->U[1 - c->ucur][f]; a.frc[f] = c->frc[f]; }
+  a.S = c->h[c->cur];
+  a.O = c->h[1 - c->cur];
+  a.uin = c->U[c->ucur];
+  a.uout = c->U[1 - c->ucur];
   a.phiS = c->phi[c->cur];
   a.phiO = c->phi[1 - c->cur];
-  for (int f = 0; f < 7; ++f) a.aux[f] = c->aux[f];
+  a.aux = c->aux;
   a.lap = c->lap;
   a.dvt = c->dvt;
+  a.frc = c->frc;
+  a.pre = c->pre;
+  a.gf_from_pre = 0;
   a.isfh = c->isfh;
   a.dp = c->dp;
   return a;
 }
 
-static FusedArgs fused_args(lbf_ctx* c, bool inl) {
+static FusedArgs fused_args(lbf_ctx* c) {
   FusedArgs a{};
-  for (int f = 0; f < 10; ++f) { a.min[f] = c->h[c->cur][f]; a.mout[f] = c->h[1 - c->cur][f]; }
-  for (int f = 0; f < 3; ++f) { a.uin[f] = c->U[c->ucur][f]; a.uout[f] = c->U[1 - c->ucur][f]; a.frc[f] = c->frc[f]; }
+  a.min = c->h[c->cur];
+  a.mout = c->h[1 - c->cur];
+  a.uin = c->U[c->ucur];
+  a.uout = c->U[1 - c->ucur];
   a.phin = c->phi[c->cur];
   a.phiout = c->phi[1 - c->cur];
-  for (int f = 0; f < 4; ++f) a.aux[f] = c->aux[f];
-  a.lap = c->lap;
-  a.dvt = c->dvt;
+  a.pre = c->pre;
+  a.frc = c->frc;
   a.isfh = c->isfh;
   a.dp = c->dp;
   a.ty = c->ty;
   a.tz = c->tz;
-  (void)inl;
   return a;
 }
 
@@ -473,21 +499,22 @@ int lbf_init_state(lbf_ctx* c) {
   const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
   if (devc) {
     for (int f = 0; f < 10; ++f)
-      if (!c->raw[f]) return fail(c, LBF_ESTATE, "moment field %d not uploaded", f + 1);
+      if (!c->raw_have[f]) return fail(c, LBF_ESTATE, "moment field %d not uploaded", f + 1);
     HIPCHK(c, launch_encode_moments(c->h[c->cur], c->raw, c->dp, c->prec, c->stream));
     c->raw_pending = false;
   }
   // O := S (init_conditions_module.f90:584-587), u_old := u
-  for (int f = 0; f < 10; ++f) HIPCHK(c, launch_copy(c->h[1 - c->cur][f], c->h[c->cur][f], c->N * c->tsz, c->stream));
+  HIPCHK(c, launch_copy(c->h[1 - c->cur], c->h[c->cur], 10 * c->N * c->tsz, c->stream));
   for (int b = 0; b < 2; ++b)
-    for (int f = 0; f < 3; ++f) HIPCHK(c, launch_copy(c->U[b][f], c->h[c->cur][1 + f], c->N * c->tsz, c->stream));
+    HIPCHK(c, launch_copy(c->U[b], (char*)c->h[c->cur] + c->N * c->tsz, 3 * c->N * c->tsz, c->stream));
   if (c->p.twocomponent) {
     HIPCHK(c, launch_copy(c->phi[1 - c->cur], c->phi[c->cur], c->N * c->asz, c->stream));
-    for (int f = 0; f < 7; ++f) HIPCHK(c, launch_fill_zero(c->aux[f], c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->aux, 7 * c->N * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->lap, c->N * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->dvt, c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->pre, 5 * c->N * c->asz, c->stream));
   }
-  for (int f = 0; f < 3; ++f) HIPCHK(c, launch_fill_zero(c->frc[f], c->N * c->asz, c->stream));
+  HIPCHK(c, launch_fill_zero(c->frc, 3 * c->N * c->asz, c->stream));
   // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0
   if (c->p.twocomponent && c->p.internal_obstacles) {
     StageArgs a = stage_args(c);
@@ -500,7 +527,7 @@ int lbf_init_state(lbf_ctx* c) {
 }
 
 static int run_fused(lbf_ctx* c, bool inl) {
-  FusedArgs a = fused_args(c, inl);
+  FusedArgs a = fused_args(c);
   if (c->nev_used + 2 > (int)c->evs.size()) {
     for (int i = 0; i < 2; ++i) {
       hipEvent_t ev;
@@ -524,10 +551,12 @@ int lbf_step(lbf_ctx* c, int32_t nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     ++c->step;
     if (c->p.twocomponent) {
+      // compute_norm + div_theta_n (one tiled pass), then the ghost-phi
+      // update on the solid nodes, then the fused step
       StageArgs a = stage_args(c);
-      HIPCHK(c, launch_stage(LBF_STAGE_NORM, a, c->prec, c->stream));
+      a.gf_from_pre = 1;
+      HIPCHK(c, launch_phase_prep(a, c->prec, c->tz_prep, c->stream));
       if (c->p.internal_obstacles) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
-      HIPCHK(c, launch_stage(LBF_STAGE_DIV_THETA_N, a, c->prec, c->stream));
     }
     int rc = run_fused(c, true);
     if (rc) return rc;
@@ -608,9 +637,7 @@ int lbf_end_step(lbf_ctx* c) {
 
 int lbf_reduce_fp64(lbf_ctx* c, double out[8]) {
   if (!c || !out) return LBF_EINVAL;
-  const void* S[10];
-  for (int f = 0; f < 10; ++f) S[f] = c->h[c->cur][f];
-  HIPCHK(c, launch_reduce(S, c->phi[c->cur], c->isfh, c->dp, c->prec, c->partial, c->nred, c->red, c->stream));
+  HIPCHK(c, launch_reduce(c->h[c->cur], c->phi[c->cur], c->isfh, c->dp, c->prec, c->partial, c->nred, c->red, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, c->red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBF_OK;

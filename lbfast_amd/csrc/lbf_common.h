@@ -7,77 +7,144 @@
 
 namespace lbf {
 
-// ---------------------------------------------------------------------------
-// Kernel parameter block (by value).  Doubles are converted to the compute
-// type R at kernel entry.
+// Physical scalars in the compute type (converted once on the host).
+template <class R>
+struct Phys {
+  R fx, fy, fz;
+  R visc1, visc2, rho_r, rho_b;
+  R tau_diff, sharp_c, beta, kapp, omega;
+  R wettab;
+};
+
+// Kernel parameter block (by value).
 struct DevParams {
   int nx, ny, nz;
   int pbc[3];
   int twocomp, dens, incomp, obst, buoy;
   int lattice, highorder;
-  double fx, fy, fz;
-  double visc1, visc2, rho_r, rho_b;
-  double tau_diff, sharp_c, beta, kapp, omega;
-  double wettab;
-  float scale[10];     // FP16S/BF16S: stored = scale * deviation
+  size_t nn;             // nx*ny*nz: stride between the fields of a field set
+  Phys<double> pd;
+  Phys<float> pf;
+  float scale[10];       // FP16S/BF16S: stored = scale * deviation
   float invscale[10];
+  template <class R> __device__ __forceinline__ const Phys<R>& ph() const;
 };
+template <> __device__ __forceinline__ const Phys<double>& DevParams::ph<double>() const { return pd; }
+template <> __device__ __forceinline__ const Phys<float>& DevParams::ph<float>() const { return pf; }
 
-__host__ __device__ inline int wrapi(int i, int n) {
-  // i in [-n, 2n)
-  return i < 0 ? i + n : (i >= n ? i - n : i);
-}
 __host__ __device__ inline size_t nidx(int x, int y, int z, int nx, int ny) {
   return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z);
 }
 
+// Load through a wave-uniform base pointer and a 32-bit byte offset (selects
+// the scalar-base + vector-offset global addressing mode).
+template <class T>
+__device__ __forceinline__ T ldb(const T* base, uint32_t boff) {
+  return *(const T*)((const char*)base + boff);
+}
+template <class T>
+__device__ __forceinline__ void stb(T* base, uint32_t boff, T v) {
+  *(T*)((char*)base + boff) = v;
+}
+
+// Buffer-resource access: a wave-uniform base (scalar registers) plus a
+// 32-bit per-lane byte offset, range-checked against num_records (an offset
+// past the plane reads 0 / drops the store instead of faulting).  Resource
+// word 3 is the CDNA (gfx9) raw-buffer format.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc mkrsrc(const void* p, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
+}
+template <class T> __device__ __forceinline__ T bld(Rsrc r, uint32_t boff);
+template <> __device__ __forceinline__ float bld<float>(Rsrc r, uint32_t boff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+}
+template <> __device__ __forceinline__ double bld<double>(Rsrc r, uint32_t boff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+}
+template <> __device__ __forceinline__ __half bld<__half>(Rsrc r, uint32_t boff) {
+  return __builtin_bit_cast(__half, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+}
+template <> __device__ __forceinline__ __hip_bfloat16 bld<__hip_bfloat16>(Rsrc r, uint32_t boff) {
+  return __builtin_bit_cast(__hip_bfloat16, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+}
+template <> __device__ __forceinline__ int8_t bld<int8_t>(Rsrc r, uint32_t boff) {
+  return (int8_t)__builtin_amdgcn_raw_buffer_load_b8(r, boff, 0, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, boff, 0, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t boff, double v) {
+  typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) V2;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, boff, 0, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t boff, __half v) {
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, boff, 0, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t boff, __hip_bfloat16 v) {
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, boff, 0, 0);
+}
+// field f of a field set at one z plane
+template <class T>
+__device__ __forceinline__ T fld(const T* plane0, size_t fs, int f, uint32_t pbytes, uint32_t boff) {
+  return bld<T>(mkrsrc(plane0 + (size_t)f * fs, pbytes), boff);
+}
+template <class T>
+__device__ __forceinline__ void fst(T* plane0, size_t fs, int f, uint32_t pbytes, uint32_t boff, T v) {
+  bst(mkrsrc(plane0 + (size_t)f * fs, pbytes), boff, v);
+}
+
+// Reciprocal: 1-ulp hardware estimate in FP32 (well inside the FP32
+// tolerance), correctly rounded division in FP64.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
 // ---------------------------------------------------------------------------
-// Storage codecs.  A codec maps the compute type R <-> the storage type T.
-// Plain codecs store the value; the deviation codecs (FP16S/BF16S) store
-// scale*value for P and u, and scale*(Pi_ab - u_a u_b) for the second
-// moments, so that all stored quantities are small deviations from the rest
-// state and land in the normal range of the 16-bit format.
+// Storage codecs: compute type R <-> storage type T.  Plain codecs store the
+// value; the deviation codecs (FP16S/BF16S) store scale*value for P and u and
+// scale*(Pi_ab - u_a u_b) for the second moments, so every stored quantity is
+// a small deviation from the rest state inside the 16-bit normal range.
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_MIXED = 2, PREC_FP16S = 3, PREC_BF16S = 4 };
 
 template <int P> struct Codec;
 template <> struct Codec<PREC_FP64> {
   typedef double T; typedef double R;
   static constexpr bool dev = false;
-  __device__ static R ld(const T* a, size_t i, float) { return a[i]; }
-  __device__ static void stv(T* a, size_t i, R v, float) { a[i] = v; }
+  __device__ static R dec(T v, float) { return v; }
+  __device__ static T enc(R v, float) { return v; }
   __device__ static R round(R v) { return v; }
 };
 template <> struct Codec<PREC_FP32> {
   typedef float T; typedef float R;
   static constexpr bool dev = false;
-  __device__ static R ld(const T* a, size_t i, float) { return a[i]; }
-  __device__ static void stv(T* a, size_t i, R v, float) { a[i] = v; }
+  __device__ static R dec(T v, float) { return v; }
+  __device__ static T enc(R v, float) { return v; }
   __device__ static R round(R v) { return v; }
 };
 template <> struct Codec<PREC_MIXED> {
   typedef float T; typedef double R;
   static constexpr bool dev = false;
-  __device__ static R ld(const T* a, size_t i, float) { return (R)a[i]; }
-  __device__ static void stv(T* a, size_t i, R v, float) { a[i] = (T)v; }
+  __device__ static R dec(T v, float) { return (R)v; }
+  __device__ static T enc(R v, float) { return (T)v; }
   // the reference stores intermediates (S after moments, forces) in strdb
   __device__ static R round(R v) { return (R)(T)v; }
 };
 template <> struct Codec<PREC_FP16S> {
   typedef __half T; typedef float R;
   static constexpr bool dev = true;
-  __device__ static R ld(const T* a, size_t i, float inv) { return __half2float(a[i]) * inv; }
-  __device__ static void stv(T* a, size_t i, R v, float s) { a[i] = __float2half_rn(v * s); }
+  __device__ static R dec(T v, float inv) { return __half2float(v) * inv; }
+  __device__ static T enc(R v, float s) { return __float2half_rn(v * s); }
   __device__ static R round(R v) { return v; }
 };
 template <> struct Codec<PREC_BF16S> {
   typedef __hip_bfloat16 T; typedef float R;
   static constexpr bool dev = true;
-  __device__ static R ld(const T* a, size_t i, float inv) { return __bfloat162float(a[i]) * inv; }
-  __device__ static void stv(T* a, size_t i, R v, float s) { a[i] = __float2bfloat16(v * s); }
+  __device__ static R dec(T v, float inv) { return __bfloat162float(v) * inv; }
+  __device__ static T enc(R v, float s) { return __float2bfloat16(v * s); }
   __device__ static R round(R v) { return v; }
 };
 
-// phase field / aux / forces / u_old storage type per precision mode
+// phase field / aux / forces storage type per precision mode
 template <int P> struct Aux { typedef typename Codec<P>::T T; };
 template <> struct Aux<PREC_FP16S> { typedef float T; };
 template <> struct Aux<PREC_BF16S> { typedef float T; };
@@ -88,23 +155,23 @@ template <class R> struct Mom {
   R P, u, v, w, xx, yy, zz, xy, xz, yz;
 };
 
-// Load/store the ten moment fields (SoA pointers m[0..9]).  For the
-// deviation codecs the second moments are stored as Pi - u u, with u taken
-// after its own rounding so that a load reproduces Pi consistently.
+// Field set = 10 fields, field f at base + f*fs.  `base` already includes any
+// wave-uniform plane offset; `boff` is the per-lane byte offset.
 template <int P>
-__device__ inline Mom<typename Codec<P>::R> load_mom(typename Codec<P>::T* const* m, size_t i, const DevParams& dp) {
+__device__ __forceinline__ Mom<typename Codec<P>::R> load_mom(const typename Codec<P>::T* base, size_t fs,
+                                                              uint32_t boff, const DevParams& dp) {
   typedef Codec<P> C; typedef typename C::R R;
   Mom<R> o;
-  o.P = C::ld(m[0], i, dp.invscale[0]);
-  o.u = C::ld(m[1], i, dp.invscale[1]);
-  o.v = C::ld(m[2], i, dp.invscale[2]);
-  o.w = C::ld(m[3], i, dp.invscale[3]);
-  o.xx = C::ld(m[4], i, dp.invscale[4]);
-  o.yy = C::ld(m[5], i, dp.invscale[5]);
-  o.zz = C::ld(m[6], i, dp.invscale[6]);
-  o.xy = C::ld(m[7], i, dp.invscale[7]);
-  o.xz = C::ld(m[8], i, dp.invscale[8]);
-  o.yz = C::ld(m[9], i, dp.invscale[9]);
+  o.P = C::dec(ldb(base + 0 * fs, boff), dp.invscale[0]);
+  o.u = C::dec(ldb(base + 1 * fs, boff), dp.invscale[1]);
+  o.v = C::dec(ldb(base + 2 * fs, boff), dp.invscale[2]);
+  o.w = C::dec(ldb(base + 3 * fs, boff), dp.invscale[3]);
+  o.xx = C::dec(ldb(base + 4 * fs, boff), dp.invscale[4]);
+  o.yy = C::dec(ldb(base + 5 * fs, boff), dp.invscale[5]);
+  o.zz = C::dec(ldb(base + 6 * fs, boff), dp.invscale[6]);
+  o.xy = C::dec(ldb(base + 7 * fs, boff), dp.invscale[7]);
+  o.xz = C::dec(ldb(base + 8 * fs, boff), dp.invscale[8]);
+  o.yz = C::dec(ldb(base + 9 * fs, boff), dp.invscale[9]);
   if (C::dev) {
     o.xx += o.u * o.u; o.yy += o.v * o.v; o.zz += o.w * o.w;
     o.xy += o.u * o.v; o.xz += o.u * o.w; o.yz += o.v * o.w;
@@ -113,42 +180,87 @@ __device__ inline Mom<typename Codec<P>::R> load_mom(typename Codec<P>::T* const
 }
 
 template <int P>
-__device__ inline void store_mom(typename Codec<P>::T* const* m, size_t i, const Mom<typename Codec<P>::R>& o,
-                                 const DevParams& dp) {
-  typedef Codec<P> C; typedef typename C::R R;
-  C::stv(m[0], i, o.P, dp.scale[0]);
+__device__ __forceinline__ void store_mom(typename Codec<P>::T* base, size_t fs, uint32_t boff,
+                                          const Mom<typename Codec<P>::R>& o, const DevParams& dp) {
+  typedef Codec<P> C; typedef typename C::R R; typedef typename C::T T;
+  stb(base + 0 * fs, boff, C::enc(o.P, dp.scale[0]));
   if (C::dev) {
-    // round u to storage first, then store Pi - u_r u_r
-    typename C::T tu, tv, tw;
-    tu = (typename C::T)0; tv = tu; tw = tu;
-    C::stv(&tu, 0, o.u, dp.scale[1]);
-    C::stv(&tv, 0, o.v, dp.scale[2]);
-    C::stv(&tw, 0, o.w, dp.scale[3]);
-    R ur = C::ld(&tu, 0, dp.invscale[1]), vr = C::ld(&tv, 0, dp.invscale[2]), wr = C::ld(&tw, 0, dp.invscale[3]);
-    m[1][i] = tu; m[2][i] = tv; m[3][i] = tw;
-    C::stv(m[4], i, o.xx - ur * ur, dp.scale[4]);
-    C::stv(m[5], i, o.yy - vr * vr, dp.scale[5]);
-    C::stv(m[6], i, o.zz - wr * wr, dp.scale[6]);
-    C::stv(m[7], i, o.xy - ur * vr, dp.scale[7]);
-    C::stv(m[8], i, o.xz - ur * wr, dp.scale[8]);
-    C::stv(m[9], i, o.yz - vr * wr, dp.scale[9]);
+    // round u first, then store Pi - u_r u_r so a load reproduces Pi
+    const T tu = C::enc(o.u, dp.scale[1]), tv = C::enc(o.v, dp.scale[2]), tw = C::enc(o.w, dp.scale[3]);
+    const R ur = C::dec(tu, dp.invscale[1]), vr = C::dec(tv, dp.invscale[2]), wr = C::dec(tw, dp.invscale[3]);
+    stb(base + 1 * fs, boff, tu); stb(base + 2 * fs, boff, tv); stb(base + 3 * fs, boff, tw);
+    stb(base + 4 * fs, boff, C::enc(o.xx - ur * ur, dp.scale[4]));
+    stb(base + 5 * fs, boff, C::enc(o.yy - vr * vr, dp.scale[5]));
+    stb(base + 6 * fs, boff, C::enc(o.zz - wr * wr, dp.scale[6]));
+    stb(base + 7 * fs, boff, C::enc(o.xy - ur * vr, dp.scale[7]));
+    stb(base + 8 * fs, boff, C::enc(o.xz - ur * wr, dp.scale[8]));
+    stb(base + 9 * fs, boff, C::enc(o.yz - vr * wr, dp.scale[9]));
   } else {
-    C::stv(m[1], i, o.u, 0.f); C::stv(m[2], i, o.v, 0.f); C::stv(m[3], i, o.w, 0.f);
-    C::stv(m[4], i, o.xx, 0.f); C::stv(m[5], i, o.yy, 0.f); C::stv(m[6], i, o.zz, 0.f);
-    C::stv(m[7], i, o.xy, 0.f); C::stv(m[8], i, o.xz, 0.f); C::stv(m[9], i, o.yz, 0.f);
+    stb(base + 1 * fs, boff, C::enc(o.u, 0.f)); stb(base + 2 * fs, boff, C::enc(o.v, 0.f));
+    stb(base + 3 * fs, boff, C::enc(o.w, 0.f));
+    stb(base + 4 * fs, boff, C::enc(o.xx, 0.f)); stb(base + 5 * fs, boff, C::enc(o.yy, 0.f));
+    stb(base + 6 * fs, boff, C::enc(o.zz, 0.f)); stb(base + 7 * fs, boff, C::enc(o.xy, 0.f));
+    stb(base + 8 * fs, boff, C::enc(o.xz, 0.f)); stb(base + 9 * fs, boff, C::enc(o.yz, 0.f));
+  }
+}
+
+// Buffer-addressed variants: plane0 = field 0 of the set at the wanted z plane.
+template <int P>
+__device__ __forceinline__ Mom<typename Codec<P>::R> load_mom_b(const typename Codec<P>::T* plane0, size_t fs,
+                                                                uint32_t pb, uint32_t boff, const DevParams& dp) {
+  typedef Codec<P> C; typedef typename C::R R;
+  Mom<R> o;
+  o.P = C::dec(fld(plane0, fs, 0, pb, boff), dp.invscale[0]);
+  o.u = C::dec(fld(plane0, fs, 1, pb, boff), dp.invscale[1]);
+  o.v = C::dec(fld(plane0, fs, 2, pb, boff), dp.invscale[2]);
+  o.w = C::dec(fld(plane0, fs, 3, pb, boff), dp.invscale[3]);
+  o.xx = C::dec(fld(plane0, fs, 4, pb, boff), dp.invscale[4]);
+  o.yy = C::dec(fld(plane0, fs, 5, pb, boff), dp.invscale[5]);
+  o.zz = C::dec(fld(plane0, fs, 6, pb, boff), dp.invscale[6]);
+  o.xy = C::dec(fld(plane0, fs, 7, pb, boff), dp.invscale[7]);
+  o.xz = C::dec(fld(plane0, fs, 8, pb, boff), dp.invscale[8]);
+  o.yz = C::dec(fld(plane0, fs, 9, pb, boff), dp.invscale[9]);
+  if (C::dev) {
+    o.xx += o.u * o.u; o.yy += o.v * o.v; o.zz += o.w * o.w;
+    o.xy += o.u * o.v; o.xz += o.u * o.w; o.yz += o.v * o.w;
+  }
+  return o;
+}
+
+template <int P>
+__device__ __forceinline__ void store_mom_b(typename Codec<P>::T* plane0, size_t fs, uint32_t pb, uint32_t boff,
+                                            const Mom<typename Codec<P>::R>& o, const DevParams& dp) {
+  typedef Codec<P> C; typedef typename C::R R; typedef typename C::T T;
+  fst(plane0, fs, 0, pb, boff, C::enc(o.P, dp.scale[0]));
+  if (C::dev) {
+    const T tu = C::enc(o.u, dp.scale[1]), tv = C::enc(o.v, dp.scale[2]), tw = C::enc(o.w, dp.scale[3]);
+    const R ur = C::dec(tu, dp.invscale[1]), vr = C::dec(tv, dp.invscale[2]), wr = C::dec(tw, dp.invscale[3]);
+    fst(plane0, fs, 1, pb, boff, tu); fst(plane0, fs, 2, pb, boff, tv); fst(plane0, fs, 3, pb, boff, tw);
+    fst(plane0, fs, 4, pb, boff, C::enc(o.xx - ur * ur, dp.scale[4]));
+    fst(plane0, fs, 5, pb, boff, C::enc(o.yy - vr * vr, dp.scale[5]));
+    fst(plane0, fs, 6, pb, boff, C::enc(o.zz - wr * wr, dp.scale[6]));
+    fst(plane0, fs, 7, pb, boff, C::enc(o.xy - ur * vr, dp.scale[7]));
+    fst(plane0, fs, 8, pb, boff, C::enc(o.xz - ur * wr, dp.scale[8]));
+    fst(plane0, fs, 9, pb, boff, C::enc(o.yz - vr * wr, dp.scale[9]));
+  } else {
+    fst(plane0, fs, 1, pb, boff, C::enc(o.u, 0.f)); fst(plane0, fs, 2, pb, boff, C::enc(o.v, 0.f));
+    fst(plane0, fs, 3, pb, boff, C::enc(o.w, 0.f));
+    fst(plane0, fs, 4, pb, boff, C::enc(o.xx, 0.f)); fst(plane0, fs, 5, pb, boff, C::enc(o.yy, 0.f));
+    fst(plane0, fs, 6, pb, boff, C::enc(o.zz, 0.f)); fst(plane0, fs, 7, pb, boff, C::enc(o.xy, 0.f));
+    fst(plane0, fs, 8, pb, boff, C::enc(o.xz, 0.f)); fst(plane0, fs, 9, pb, boff, C::enc(o.yz, 0.f));
   }
 }
 
 // ---------------------------------------------------------------------------
-// Lattice.  Velocity ordering of the reference (vars_module.f90:319-338; the
-// first 19 are the D3Q19 set, 666-684).  Kernels address populations by
-// their velocity (p,q,r) in {-1,0,1}^3 through cube index c = (p+1) +
-// 3(q+1) + 9(r+1).
+// Lattice.  Populations are addressed by velocity (p,q,r) in {-1,0,1}^3
+// through cube index c = (p+1) + 3(q+1) + 9(r+1); the reference's ordering
+// (vars_module.f90:319-338, D3Q19 = first 19 entries) matters only for
+// loops that follow its order, which use their own tables.
 __host__ __device__ constexpr int cidx(int p, int q, int r) { return (p + 1) + 3 * (q + 1) + 9 * (r + 1); }
 
 template <int LAT>
 __host__ __device__ constexpr bool present(int p, int q, int r) {
-  // D3Q27: all 27; D3Q19: drop the 8 corners (|p|+|q|+|r| == 3)
+  // D3Q27: all 27; D3Q19: drop the 8 corners
   return LAT == 27 ? true : ((p != 0 ? 1 : 0) + (q != 0 ? 1 : 0) + (r != 0 ? 1 : 0)) < 3;
 }
 template <int LAT>

