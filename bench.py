@@ -122,6 +122,19 @@ def cpu_baseline(target_s=15.0):
                       f"droplet 64^3, {steps} steps in {t:.1f} s"}
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the fused kernel measured with rocprofv3
+    counters (profiles/hbm_traffic.txt), or None."""
+    fn = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(fn):
+        return None
+    for line in open(fn):
+        parts = line.split()
+        if len(parts) == 4 and not line.startswith("#") and parts[0] == workload:
+            return int(parts[2]) + int(parts[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,7 +145,20 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage legs")
     ap.add_argument("--only", default="", help="run one leg: fp32 | fp16s | bf16s | 1c")
     ap.add_argument("--tile", default="", help="ty,tz override of the fused kernel tile")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 28  # 2^28 float32 = 1 GiB read + 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU sharding is implemented); "
@@ -188,7 +214,8 @@ def main():
                    "baseline": "1x A100 2c 512^3 D3Q27h SP 800 MLUPS (README.md:405)"},
         "roofline": {"bound": "hbm", "achieved": round(main_leg["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_leg["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "fused_kernel (inline moments)",
+                     "traffic": measured_traffic(f"laplace_droplet_2c_d3q27h_densratio_{args.n}^3"),
+                     "kernel": "fused_kernel (inline moments)",
                      "bytes_per_node": BYTES_PER_LUP[PREC_FP32], "kernel_ms": round(main_leg["fused_ms"], 4)},
         "hbm_frac_step": round(main_leg["step_gbs"] / HBM_PEAK_GBS, 4),
     }

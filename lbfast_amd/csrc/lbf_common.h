@@ -94,6 +94,17 @@ __device__ __forceinline__ void fst(T* plane0, size_t fs, int f, uint32_t pbytes
   bst(mkrsrc(plane0 + (size_t)f * fs, pbytes), boff, v);
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (workgroup b runs on XCD b % 8), each with its own L2.  Give XCD k a
+// contiguous range of tiles in (x fastest, y, z) order, so the tiles resident
+// together on one XCD are x- and y-neighbours that share halo cache lines.
+constexpr int kNumXCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int per = ntiles / kNumXCD, rem = ntiles % kNumXCD;
+  const int k = b % kNumXCD, slot = b / kNumXCD;
+  return k * per + (k < rem ? k : rem) + slot;
+}
+
 // Reciprocal: 1-ulp hardware estimate in FP32 (well inside the FP32
 // tolerance), correctly rounded division in FP64.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -166,10 +166,14 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nn = dp.nn, nxny = (size_t)nx * ny;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int x0 = blockIdx.x * 64;
+  // 1-D grid, XCD-aware tile order (tx fastest)
+  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY, ntz = (nz + a.tz - 1) / a.tz;
+  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tzi = tile / (ntx * nty);
+  const int x0 = tx * 64;
   const int XW = min(64, nx - x0);
-  const int y0 = blockIdx.y * TY;
-  const int z0 = blockIdx.z * a.tz;
+  const int y0 = ty * TY;
+  const int z0 = tzi * a.tz;
   const int zend = min(z0 + a.tz, nz);
   const bool halo_wave = (wave == NW);
 
@@ -412,7 +416,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
 // ---------------------------------------------------------------------------
 template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
 static hipError_t launch_t(const FusedArgs& a, hipStream_t s) {
-  dim3 grid((a.dp.nx + 63) / 64, (a.dp.ny + TY - 1) / TY, (a.dp.nz + a.tz - 1) / a.tz);
+  dim3 grid(((a.dp.nx + 63) / 64) * ((a.dp.ny + TY - 1) / TY) * ((a.dp.nz + a.tz - 1) / a.tz));
   dim3 block((TY + 3) * 64);
   hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY>), grid, block, 0, s, a);
   return hipGetLastError();

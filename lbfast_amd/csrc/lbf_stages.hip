@@ -154,8 +154,10 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nn = dp.nn, nxny = (size_t)nx * ny;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * PTY;
-  const int z0 = blockIdx.z * tz, zend = min(z0 + tz, nz);
+  const int ntx = (nx + PTX - 1) / PTX, nty = (ny + PTY - 1) / PTY, ntz = (nz + tz - 1) / tz;
+  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int x0 = (tile % ntx) * PTX, y0 = ((tile / ntx) % nty) * PTY;
+  const int z0 = (tile / (ntx * nty)) * tz, zend = min(z0 + tz, nz);
   const TA* phi = (const TA*)a.phiS;
   const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
   const R THREE = 3, ONE = 1, TWO = 2;
@@ -485,7 +487,7 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
 
 template <int PREC>
 static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
-  const dim3 grid((a.dp.nx + PTX - 1) / PTX, (a.dp.ny + PTY - 1) / PTY, (a.dp.nz + tz - 1) / tz);
+  const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
   hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
   return hipGetLastError();
 }
