@@ -24,6 +24,10 @@
  *   update_phifields                (driver:1520-1899) -> lbf_stage(LBF_STAGE_UPDATE_PHI)
  *   LB_int_boundary_cuda            (driver:1901-1963) -> lbf_stage(LBF_STAGE_LB_INT_BOUNDARY)
  *   PHI_int_boundary_cuda           (driver:1965-2009) -> lbf_stage(LBF_STAGE_PHI_INT_BOUNDARY)
+ *   thinfilm_scan_mark_cuda         (driver:850-880)   -> lbf_stage(LBF_STAGE_THINFILM_SCAN)
+ *   repulsive_flux_normal_cuda      (driver:882-910)   -> lbf_stage(LBF_STAGE_REPULSIVE_FLUX)
+ *   fused_LB_cuda under BOUNCE_BACK (driver:1464-1465, fused_LB_kernel_bb)
+ *                                                      -> lbf_stage(LBF_STAGE_FUSED), bounce_back = 1
  *   exchange_*_intpbc (mpi_module.f90:2203-3524)       -> in-kernel periodic wrap (no call)
  *   integrator time loop (integrator_module.f90:266-827) -> lbf_step (fused fast path)
  *   initial O:=S copy (init_conditions_module.f90:584-587) + bcs_phi
@@ -86,7 +90,12 @@ extern "C" {
 #define LBF_FIELD_PHI_O   37  /* the other (O) phase-field buffer              */
 #define LBF_FIELD_RHOPHI  40  /* observable rho_r phi + (1-phi) rho_b (download only;
                                  print_module.f90:734-760)                     */
-#define LBF_NFIELD_IDS    41
+#define LBF_FIELD_LOCAUX3 41  /* locaux 3..5: facing-partner indices (1-based,  */
+#define LBF_FIELD_LOCAUX5 43  /*  unwrapped; lb_cuda_repulsive_module.f90:144-146) */
+#define LBF_FIELD_LOCAUX6 44  /* locaux 6..8: repulsive flux (repulsive:285-287) */
+#define LBF_FIELD_LOCAUX8 46
+#define LBF_FIELD_REP_MASK 47 /* rep_mask (0/1, download only)                 */
+#define LBF_NFIELD_IDS    48
 
 /* ---- stage ids (lbf_stage) --------------------------------------------- */
 #define LBF_STAGE_NORM              1  /* compute_norm_interface_kernel    */
@@ -96,6 +105,8 @@ extern "C" {
 #define LBF_STAGE_FUSED             5  /* fused_LB_kernel                  */
 #define LBF_STAGE_UPDATE_PHI        6  /* update_phifields_kernel          */
 #define LBF_STAGE_LB_INT_BOUNDARY   7  /* LB_int_boundary_kernel           */
+#define LBF_STAGE_THINFILM_SCAN     8  /* thinfilm_scan_mark_kernel (REPULSIVE_FLUX) */
+#define LBF_STAGE_REPULSIVE_FLUX    9  /* repulsive_flux_normal_kernel     */
 
 /* Parameters: the &simulation / &fluid namelists (LBFAST.f90:32-60) that
  * the time step reads, the defines.h switches, and the derived quantities
@@ -111,7 +122,10 @@ typedef struct lbf_params {
   int32_t buoyancy_forcing;      /* BUOYANCY_FORCING                        */
   int32_t pbc[3];                /* pbc_x pbc_y pbc_z (1 periodic, 0 wall)  */
   int32_t precision;             /* LBF_PREC_*                              */
-  int32_t reserved_i[4];         /* zero                                    */
+  int32_t bounce_back;           /* BOUNCE_BACK (with internal_obstacles)    */
+  int32_t smagorinsky;           /* SMAGORINSKI                             */
+  int32_t repulsive_flux;        /* REPULSIVE_FLUX (two-component)          */
+  int32_t reserved_i;            /* zero                                    */
   double fx, fy, fz;             /* body force                              */
   double visc1, visc2;           /* kinematic viscosities                   */
   double rho_r, rho_b;           /* phase densities (DENSRATIO)             */
@@ -121,7 +135,9 @@ typedef struct lbf_params {
   double tau1, omega, tau2, beta, kapp, sharp_c;
   /* FP16S/BF16S: stored value = half(scale * deviation); 0 selects default */
   double store_scale[10];
-  double reserved_d[4];
+  /* REPULSIVE_FLUX namelist values (LBFAST.f90:57); the scan window is 3 (LBFAST.f90:221) */
+  double a_rep, q_th, cos_opp_t;
+  double reserved_d;
 } lbf_params;
 
 typedef struct lbf_ctx lbf_ctx;

@@ -36,6 +36,8 @@ struct lbf_ctx {
   void* frc = nullptr;  // 3-field set (stage path)
   void* pre = nullptr;  // 5-field set (fast path phase-field prep)
   int8_t* isfh = nullptr;
+  void* rep = nullptr;         // REPULSIVE_FLUX: locaux 3..8 (6-field set)
+  int8_t* rep_mask = nullptr;  // REPULSIVE_FLUX
   float* raw = nullptr;  // 10-field FP32 staging of 16-bit storage uploads
   bool raw_have[10] = {};
   void* staging = nullptr;
@@ -179,6 +181,9 @@ static void fill_devparams(lbf_ctx* c) {
   d.buoy = p.buoyancy_forcing;
   d.lattice = p.lattice;
   d.highorder = p.highorder && p.lattice == 27;
+  d.bb = p.internal_obstacles && p.bounce_back;
+  d.smag = p.smagorinsky != 0;
+  d.rep = p.twocomponent && p.repulsive_flux;
   d.nn = (size_t)p.nx * p.ny * p.nz;
   // without DENSRATIO the reference's viscosity mix still reads rho_r/rho_b
   Phys<double>& q = d.pd;
@@ -186,11 +191,13 @@ static void fill_devparams(lbf_ctx* c) {
   q.visc1 = p.visc1; q.visc2 = p.visc2; q.rho_r = p.rho_r; q.rho_b = p.rho_b;
   q.tau_diff = p.tau_diff; q.sharp_c = p.sharp_c; q.beta = p.beta; q.kapp = p.kapp;
   q.omega = p.omega; q.wettab = p.wettab_r;
+  q.width = p.width; q.a_rep = p.a_rep; q.q_th = p.q_th; q.cos_opp = p.cos_opp_t;
   Phys<float>& f = d.pf;
   f.fx = (float)q.fx; f.fy = (float)q.fy; f.fz = (float)q.fz;
   f.visc1 = (float)q.visc1; f.visc2 = (float)q.visc2; f.rho_r = (float)q.rho_r; f.rho_b = (float)q.rho_b;
   f.tau_diff = (float)q.tau_diff; f.sharp_c = (float)q.sharp_c; f.beta = (float)q.beta; f.kapp = (float)q.kapp;
   f.omega = (float)q.omega; f.wettab = (float)q.wettab;
+  f.width = (float)q.width; f.a_rep = (float)q.a_rep; f.q_th = (float)q.q_th; f.cos_opp = (float)q.cos_opp;
   // FP16S/BF16S: power-of-two scales that put typical deviations (P ~ 1e-4..1e-2,
   // u ~ 1e-3..1e-1, Pi - u u ~ 1e-7..1e-3) in the normal range of binary16
   const double dflt[10] = {16384.0, 1024.0, 1024.0, 1024.0, 1048576.0, 1048576.0, 1048576.0,
@@ -246,6 +253,9 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   if (p.highorder && p.lattice != 27) return fail(nullptr, LBF_EINVAL, "HIGHORDER requires LATTICE 27");
   if (p.densratio && !p.twocomponent) return fail(nullptr, LBF_EINVAL, "DENSRATIO requires TWOCOMPONENT");
   if (p.precision < LBF_PREC_FP64 || p.precision > LBF_PREC_BF16S) return fail(nullptr, LBF_EINVAL, "bad precision");
+  if (p.repulsive_flux && !p.twocomponent) return fail(nullptr, LBF_EINVAL, "REPULSIVE_FLUX requires TWOCOMPONENT");
+  if (p.repulsive_flux && (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S))
+    return fail(nullptr, LBF_EINVAL, "REPULSIVE_FLUX runs the stage path: FP64, FP32 or MIXED storage");
   if (p.visc1 <= 0 || (p.twocomponent && p.visc2 <= 0)) return fail(nullptr, LBF_EINVAL, "viscosity must be > 0");
   if (p.twocomponent && p.densratio && (p.rho_r <= 0 || p.rho_b <= 0)) return fail(nullptr, LBF_EINVAL, "rho_r, rho_b must be > 0");
   if (lbf_params_derive(&p) != LBF_OK) return fail(nullptr, LBF_EINVAL, "width must be non-zero for TWOCOMPONENT");
@@ -284,6 +294,10 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     c->pre = A(5 * c->N * c->asz);
   }
   c->frc = A(3 * c->N * c->asz);
+  if (c->dp.rep) {
+    c->rep = A(6 * c->N * c->asz);
+    c->rep_mask = (int8_t*)A(c->N);
+  }
   c->isfh = (int8_t*)A((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2));
   c->staging = A(c->N * 8);
   c->nred = 1024;
@@ -351,6 +365,7 @@ static void* field_ptr(lbf_ctx* c, int id) {
   if (id == LBF_FIELD_DIVTHN) return c->dvt;
   if (id >= 27 && id <= 36) return fset(c->h[1 - c->cur], id - 27, N, T);
   if (id == LBF_FIELD_PHI_O) return c->phi[1 - c->cur];
+  if (id >= LBF_FIELD_LOCAUX3 && id <= LBF_FIELD_LOCAUX8) return fset(c->rep, id - LBF_FIELD_LOCAUX3, N, A);
   return nullptr;
 }
 
@@ -404,6 +419,17 @@ int lbf_download_field(lbf_ctx* c, int id, void* host, int dtype) {
     }
     if (dtype == 0) memcpy(host, ph.data(), bytes);
     else for (size_t n = 0; n < c->N; ++n) ((float*)host)[n] = (float)ph[n];
+    return LBF_OK;
+  }
+  if (id == LBF_FIELD_REP_MASK) {
+    if (!c->rep_mask) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
+    std::vector<int8_t> m(c->N);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->rep_mask, c->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t n = 0; n < c->N; ++n) {
+      if (dtype == 0) ((double*)host)[n] = m[n];
+      else ((float*)host)[n] = m[n];
+    }
     return LBF_OK;
   }
   void* src = field_ptr(c, id);
@@ -472,6 +498,8 @@ static StageArgs stage_args(lbf_ctx* c) {
   a.frc = c->frc;
   a.pre = c->pre;
   a.gf_from_pre = 0;
+  a.rep = c->rep;
+  a.rep_mask = c->rep_mask;
   a.isfh = c->isfh;
   a.dp = c->dp;
   return a;
@@ -515,6 +543,10 @@ int lbf_init_state(lbf_ctx* c) {
     HIPCHK(c, launch_fill_zero(c->pre, 5 * c->N * c->asz, c->stream));
   }
   HIPCHK(c, launch_fill_zero(c->frc, 3 * c->N * c->asz, c->stream));
+  if (c->rep) {
+    HIPCHK(c, launch_fill_zero(c->rep, 6 * c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->rep_mask, c->N, c->stream));
+  }
   // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0
   if (c->p.twocomponent && c->p.internal_obstacles) {
     StageArgs a = stage_args(c);
@@ -542,6 +574,30 @@ static int run_fused(lbf_ctx* c, bool inl) {
   return LBF_OK;
 }
 
+// One time step as the reference's stage sequence (integrator_module.f90:266-545),
+// stream-ordered, no host synchronisation.
+static int staged_step(lbf_ctx* c) {
+  const bool two = c->p.twocomponent != 0, obst = c->p.internal_obstacles != 0;
+  StageArgs a = stage_args(c);
+  if (two) {
+    HIPCHK(c, launch_stage(LBF_STAGE_NORM, a, c->prec, c->stream));
+    if (obst) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+    HIPCHK(c, launch_stage(LBF_STAGE_DIV_THETA_N, a, c->prec, c->stream));
+    if (c->dp.rep) {
+      HIPCHK(c, launch_stage(LBF_STAGE_THINFILM_SCAN, a, c->prec, c->stream));
+      HIPCHK(c, launch_stage(LBF_STAGE_REPULSIVE_FLUX, a, c->prec, c->stream));
+    }
+  }
+  HIPCHK(c, launch_stage(LBF_STAGE_MOMENTS, a, c->prec, c->stream));
+  int rc = run_fused(c, false);
+  if (rc) return rc;
+  if (two) HIPCHK(c, launch_stage(LBF_STAGE_UPDATE_PHI, a, c->prec, c->stream));
+  if (obst && !c->p.bounce_back) HIPCHK(c, launch_stage(LBF_STAGE_LB_INT_BOUNDARY, a, c->prec, c->stream));
+  c->cur ^= 1;
+  c->ucur ^= 1;
+  return LBF_OK;
+}
+
 int lbf_step(lbf_ctx* c, int32_t nsteps) {
   if (!c) return LBF_EINVAL;
   if (!c->initialized) return fail(c, LBF_ESTATE, "lbf_init_state must be called before lbf_step");
@@ -550,6 +606,13 @@ int lbf_step(lbf_ctx* c, int32_t nsteps) {
   HIPCHK(c, hipEventRecord(c->t0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
     ++c->step;
+    if (c->dp.rep) {
+      // the repulsive scan reads the normals of compute_norm's aux set, which
+      // the fused fast path does not form: run the reference's stage order
+      int rc = staged_step(c);
+      if (rc) return rc;
+      continue;
+    }
     if (c->p.twocomponent) {
       // compute_norm + div_theta_n (one tiled pass), then the ghost-phi
       // update on the solid nodes, then the fused step
@@ -606,9 +669,13 @@ int lbf_stage(lbf_ctx* c, int id) {
       if (!two || !obst) return LBF_OK;
       break;
     case LBF_STAGE_LB_INT_BOUNDARY:
-      if (!obst) return LBF_OK;
+      if (!obst || c->p.bounce_back) return LBF_OK;  // boundary_cds_module.f90:49-56
       break;
     case LBF_STAGE_MOMENTS:
+      break;
+    case LBF_STAGE_THINFILM_SCAN:
+    case LBF_STAGE_REPULSIVE_FLUX:
+      if (!c->dp.rep) return LBF_OK;
       break;
     case LBF_STAGE_FUSED: {
       if (c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S)

@@ -14,6 +14,7 @@ struct Phys {
   R visc1, visc2, rho_r, rho_b;
   R tau_diff, sharp_c, beta, kapp, omega;
   R wettab;
+  R width, a_rep, q_th, cos_opp;  // REPULSIVE_FLUX
 };
 
 // Kernel parameter block (by value).
@@ -22,6 +23,7 @@ struct DevParams {
   int pbc[3];
   int twocomp, dens, incomp, obst, buoy;
   int lattice, highorder;
+  int bb, smag, rep;     // BOUNCE_BACK (with obstacles), SMAGORINSKI, REPULSIVE_FLUX
   size_t nn;             // nx*ny*nz: stride between the fields of a field set
   Phys<double> pd;
   Phys<float> pf;
@@ -93,6 +95,35 @@ template <class T>
 __device__ __forceinline__ void fst(T* plane0, size_t fs, int f, uint32_t pbytes, uint32_t boff, T v) {
   bst(mkrsrc(plane0 + (size_t)f * fs, pbytes), boff, v);
 }
+
+// LDS DMA.  buffer_load_dword with the lds modifier writes each lane's dword
+// to LDS at M0 + 4*lane instead of to a VGPR: a wave can stream the next
+// plane's inputs into its own LDS slots while it computes, at no register
+// cost.  Issued from inline asm so that the compiler's wait-count tracking
+// (which would otherwise make every later s_barrier wait for the transfer)
+// does not see it; the consumer waits with dma_wait_all() before reading.
+// M0 is saved and restored around the load, and s_nop 4 covers the
+// M0-write and SGPR-write hazards ahead of the memory instruction.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ uint32_t lds_addr(float* p) {
+  return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_float*)p);
+}
+__device__ __forceinline__ void dma_dword(Rsrc r, uint32_t voff, uint32_t lds) {
+  uint32_t save;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %3\n\t"
+      "s_nop 4\n\t"
+      "buffer_load_dword %1, %2, 0 offen lds\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(save)
+      : "v"(voff), "s"(r), "s"(lds)
+      : "memory");
+}
+// Wait for every outstanding vector-memory operation of this wave (the
+// counter also covers stores, so issue a plane's stores before its DMA).
+__device__ __forceinline__ void dma_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void lds_wait_all() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs (workgroup b runs on XCD b % 8), each with its own L2.  Give XCD k a

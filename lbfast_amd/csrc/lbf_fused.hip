@@ -26,7 +26,7 @@
 // axis too.  A block owns a 64-wide x segment (one wave per y row, lanes =
 // x) of TY output rows plus one halo row on each side, and marches z.  For
 // each source plane it:
-//   x: shifts the p = +-1 populations one lane (ds_bpermute) and contracts
+//   x: shifts the p = +-1 populations one lane (DPP) and contracts
 //      over p with (1, c, c^2-1/3); the two x-halo nodes of every row are
 //      computed by one extra wave and handed over through LDS;
 //   y: exchanges the q = +-1 partial sums with the neighbouring rows through
@@ -139,16 +139,128 @@ __device__ __forceinline__ void collide(const NodeIn<R>& n, R* f) {
   }
 }
 
-template <class R>
-__device__ __forceinline__ R shfl_from_lower(R v) {  // lane L receives lane L-1
-  return __shfl_up(v, 1, 64);
+// Whole-wave lane shifts through DPP (wave_shr:1 / wave_shl:1, a VALU
+// operand modifier, no LDS round trip).  With bound_ctrl off, the lane whose
+// source lies outside the wave (lane 0 for shr, lane 63 for shl) keeps `fill`.
+__device__ __forceinline__ int dpp_shr1(int fill, int v) { return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_shl1(int fill, int v) { return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ float shift_from_lower(float v, float fill) {  // lane L receives lane L-1
+  return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(int, fill), __builtin_bit_cast(int, v)));
 }
-template <class R>
-__device__ __forceinline__ R shfl_from_upper(R v) {  // lane L receives lane L+1
-  return __shfl_down(v, 1, 64);
+__device__ __forceinline__ float shift_from_upper(float v, float fill) {  // lane L receives lane L+1
+  return __builtin_bit_cast(float, dpp_shl1(__builtin_bit_cast(int, fill), __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double shift_from_lower(double v, double fill) {
+  const int2 t = __builtin_bit_cast(int2, v), u = __builtin_bit_cast(int2, fill);
+  return __builtin_bit_cast(double, make_int2(dpp_shr1(u.x, t.x), dpp_shr1(u.y, t.y)));
+}
+__device__ __forceinline__ double shift_from_upper(double v, double fill) {
+  const int2 t = __builtin_bit_cast(int2, v), u = __builtin_bit_cast(int2, fill);
+  return __builtin_bit_cast(double, make_int2(dpp_shl1(u.x, t.x), dpp_shl1(u.y, t.y)));
+}
+// value of lane l (wave-uniform result)
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int2 t = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(t.x, l), __builtin_amdgcn_readlane(t.y, l)));
+}
+
+// Encoded inputs of one node for one source plane.
+template <int PREC>
+struct RawNode {
+  typename Codec<PREC>::T m[10];
+  typename Codec<PREC>::T uo[3];
+  typename Aux<PREC>::T phi;
+  typename Aux<PREC>::T g[5];  // inline: grad phi (3), lap phi, div theta n; stage path: forces (3)
+};
+
+// LDS slots of the prefetched inputs (one 64-lane row of dwords each):
+// moments 0-9, phi 10, g 11-15, u_old 16-18.
+template <bool TWO, bool INLINE>
+struct Slots {
+  static constexpr int PHI = 10, G = 11, UO = 16;
+  static constexpr int N = INLINE ? (TWO ? 19 : 10) : 14;
+};
+
+// Direct loads of one plane's inputs (used when the inputs are not prefetched).
+template <int PREC, bool TWO, bool INLINE>
+__device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, size_t zoff, uint32_t pbT,
+                                         uint32_t pbA, uint32_t bT, uint32_t bA) {
+  typedef typename Codec<PREC>::T T;
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  const size_t nn = dp.nn;
+  const T* S = (const T*)a.min + zoff;
+#pragma unroll
+  for (int f = 0; f < 10; ++f) r.m[f] = fld(S, nn, f, pbT, bT);
+  if constexpr (TWO) r.phi = fld((const TA*)a.phin + zoff, nn, 0, pbA, bA);
+  if constexpr (!INLINE) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) r.g[f] = fld((const TA*)a.frc + zoff, nn, f, pbA, bA);
+  } else if constexpr (TWO) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) r.g[f] = fld((const TA*)a.pre + zoff, nn, f, pbA, bA);
+    if (dp.dens) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) r.uo[f] = fld((const T*)a.uin + zoff, nn, f, pbT, bT);
+    }
+  }
+}
+
+// LDS-DMA prefetch of one plane's inputs into this wave's slots (FP32 storage).
+template <bool TWO, bool INLINE>
+__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, uint32_t pb, uint32_t boff, uint32_t lds0) {
+  typedef Slots<TWO, INLINE> SL;
+  const DevParams& dp = a.dp;
+  const size_t nn = dp.nn;
+  const float* S = (const float*)a.min + zoff;
+#pragma unroll
+  for (int f = 0; f < 10; ++f) dma_dword(mkrsrc(S + (size_t)f * nn, pb), boff, lds0 + 256u * f);
+  if constexpr (TWO) dma_dword(mkrsrc((const float*)a.phin + zoff, pb), boff, lds0 + 256u * SL::PHI);
+  if constexpr (!INLINE) {
+    const float* fr = (const float*)a.frc + zoff;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) dma_dword(mkrsrc(fr + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::G + f));
+  } else if constexpr (TWO) {
+    const float* pr = (const float*)a.pre + zoff;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) dma_dword(mkrsrc(pr + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::G + f));
+    if (dp.dens) {
+      const float* uo = (const float*)a.uin + zoff;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) dma_dword(mkrsrc(uo + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::UO + f));
+    }
+  }
+}
+
+template <bool TWO, bool INLINE>
+__device__ __forceinline__ void lds_raw(RawNode<PREC_FP32>& r, const float (*slot)[64], int lane, bool dens) {
+  typedef Slots<TWO, INLINE> SL;
+#pragma unroll
+  for (int f = 0; f < 10; ++f) r.m[f] = slot[f][lane];
+  if constexpr (TWO) r.phi = slot[SL::PHI][lane];
+  if constexpr (!INLINE) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) r.g[f] = slot[SL::G + f][lane];
+  } else if constexpr (TWO) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) r.g[f] = slot[SL::G + f][lane];
+    if (dens) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) r.uo[f] = slot[SL::UO + f][lane];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
+// PF: the next plane's inputs stream into LDS (dma_raw) while the current
+// plane is collided and streamed.  All stores of an iteration are issued
+// before that DMA, so the wait at the top of the next iteration (which the
+// shared load/store counter makes a wait for everything) finds the older
+// stores long done.  The completed output plane is therefore stored at the
+// top of the following iteration.
 template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
 __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   typedef Codec<PREC> C;
@@ -156,16 +268,19 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   typedef typename C::T T;
   typedef typename Aux<PREC>::T TA;
   constexpr int NW = TY + 2;  // row waves (TY output rows + 2 halo rows)
+  constexpr bool PF = PREC == PREC_FP32;
+  typedef Slots<TWO, INLINE> SL;
 
   __shared__ R sUP[NW][9][64];
   __shared__ R sDN[NW][9][64];
   __shared__ R sHX[NW][2][9];
+  __shared__ float sPF[PF ? NW + 1 : 1][PF ? SL::N : 1][64];
 
   const DevParams& dp = a.dp;
   const Phys<R>& ph = dp.ph<R>();
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nn = dp.nn, nxny = (size_t)nx * ny;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   // 1-D grid, XCD-aware tile order (tx fastest)
   const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY, ntz = (nz + a.tz - 1) / a.tz;
   const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -190,7 +305,8 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
     xs = side ? x0 + XW : x0 - 1;
   }
   const int ys = y0 - 1 + w;
-  const bool out_row = !halo_wave && w >= 1 && w <= TY && ys < ny;
+  // row waves: w == wave (wave-uniform); the halo wave's lanes map to (row, side)
+  const bool out_row = !halo_wave && wave >= 1 && wave <= TY && y0 - 1 + wave < ny;
   const bool out_node = out_row && lane < XW;
   const int xw = ((xs % nx) + nx) % nx;  // lanes past a short row wrap more than once
   const int yw = ((ys % ny) + ny) % ny;  // ys may exceed 2ny when ny < TY
@@ -202,52 +318,90 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   const uint32_t bI = (uint32_t)(clampi(xs, -1, nx) + 1) + (uint32_t)(nx + 2) * (uint32_t)(clampi(ys, -1, ny) + 1);
   const size_t hplane = (size_t)(nx + 2) * (ny + 2);
   const uint32_t pbI = (uint32_t)hplane;
+  auto isf_load = [&](int z) -> int {
+    return (int)bld<int8_t>(mkrsrc(a.isfh + (size_t)(clampi(z, -1, nz) + 1) * hplane, pbI), bI);
+  };
 
-  const T* Sin = (const T*)a.min;
   T* Sout = (T*)a.mout;
 
   R lo[10], mid[10], hi[10];
 #pragma unroll
   for (int m = 0; m < 10; ++m) { lo[m] = 0; mid[m] = 0; hi[m] = 0; }
+  auto store_lo = [&](int zo) {
+    Mom<R> o;
+    o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
+    o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
+    store_mom_b<PREC>(Sout + (size_t)zo * nxny, nn, pbT, bT, o, dp);
+  };
 
   const R third = (R)(1.0 / 3.0), twothird = (R)(2.0 / 3.0);
+
+  uint32_t lds0 = 0;
+  int isf_next = 1;
+  if constexpr (PF) {
+    lds0 = lds_addr(&sPF[wave][0][0]);
+    dma_raw<TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, pbT, bT, lds0);
+    if (dp.obst) isf_next = isf_load(z0 - 1);
+  }
 
   for (int zsrc = z0 - 1; zsrc <= zend; ++zsrc) {
     const int zw = wrapg(zsrc, nz);
     const size_t zoff = (size_t)zw * nxny;
-    const int8_t isf = bld<int8_t>(mkrsrc(a.isfh + (size_t)(clampi(zsrc, -1, nz) + 1) * hplane, pbI), bI);
     const bool own_plane = zsrc >= z0 && zsrc < zend;
 
-    // ---- 1. load + (inline) moments stage + collision -----------------------
+    RawNode<PREC> cur;
+    int8_t isf = 1;
+    if constexpr (PF) {
+      dma_wait_all();
+      int t = isf_next;
+      asm volatile("" : "+v"(t));  // consume the prefetched isfluid before this iteration's stores
+      isf = (int8_t)t;
+      // the plane completed by the previous iteration
+      if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2);
+#pragma unroll
+      for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+      lds_raw<TWO, INLINE>(cur, sPF[wave], lane, dp.dens);
+    } else {
+      if (dp.obst) isf = (int8_t)isf_load(zsrc);
+      load_raw<PREC, TWO, INLINE>(cur, a, zoff, pbT, pbA, bT, bA);
+    }
+
+    // ---- 1. decode + (inline) moments stage + collision ---------------------
     R f[27];
     R lbc[10];
     bool has_lbc = false;
     {
       NodeIn<R> in;
-      Mom<R> m = load_mom_b<PREC>(Sin + zoff, nn, pbT, bT, dp);
+      Mom<R> m;
+      m.P = C::dec(cur.m[0], dp.invscale[0]); m.u = C::dec(cur.m[1], dp.invscale[1]);
+      m.v = C::dec(cur.m[2], dp.invscale[2]); m.w = C::dec(cur.m[3], dp.invscale[3]);
+      m.xx = C::dec(cur.m[4], dp.invscale[4]); m.yy = C::dec(cur.m[5], dp.invscale[5]);
+      m.zz = C::dec(cur.m[6], dp.invscale[6]); m.xy = C::dec(cur.m[7], dp.invscale[7]);
+      m.xz = C::dec(cur.m[8], dp.invscale[8]); m.yz = C::dec(cur.m[9], dp.invscale[9]);
+      if (C::dev) {
+        m.xx += m.u * m.u; m.yy += m.v * m.v; m.zz += m.w * m.w;
+        m.xy += m.u * m.v; m.xz += m.u * m.w; m.yz += m.v * m.w;
+      }
       R phl = 0, rinv = (R)1, Fx, Fy, Fz;
       const bool solid = dp.obst && isf == 0;
-      if constexpr (TWO) phl = (R)fld((const TA*)a.phin + zoff, nn, 0, pbA, bA);
+      if constexpr (TWO) phl = (R)cur.phi;
       if (TWO && dp.dens) rinv = rcp(ph.rho_r * phl + ((R)1 - phl) * ph.rho_b);
       if constexpr (!INLINE) {
-        const TA* fr = (const TA*)a.frc + zoff;
-        Fx = (R)fld(fr, nn, 0, pbA, bA) * rinv;
-        Fy = (R)fld(fr, nn, 1, pbA, bA) * rinv;
-        Fz = (R)fld(fr, nn, 2, pbA, bA) * rinv;
+        Fx = (R)cur.g[0] * rinv;
+        Fy = (R)cur.g[1] * rinv;
+        Fz = (R)cur.g[2] * rinv;
       } else {
         R gfx = 0, gfy = 0, gfz = 0, lap = 0, dvt = 0, uox = 0, uoy = 0, uoz = 0;
         if constexpr (TWO) {
-          const TA* pr = (const TA*)a.pre + zoff;
-          gfx = (R)fld(pr, nn, 0, pbA, bA);
-          gfy = (R)fld(pr, nn, 1, pbA, bA);
-          gfz = (R)fld(pr, nn, 2, pbA, bA);
-          lap = (R)fld(pr, nn, 3, pbA, bA);
-          dvt = (R)fld(pr, nn, 4, pbA, bA);
+          gfx = (R)cur.g[0];
+          gfy = (R)cur.g[1];
+          gfz = (R)cur.g[2];
+          lap = (R)cur.g[3];
+          dvt = (R)cur.g[4];
           if (dp.dens) {
-            const T* uo = (const T*)a.uin + zoff;
-            uox = C::dec(fld(uo, nn, 0, pbT, bT), dp.invscale[1]);
-            uoy = C::dec(fld(uo, nn, 1, pbT, bT), dp.invscale[2]);
-            uoz = C::dec(fld(uo, nn, 2, pbT, bT), dp.invscale[3]);
+            uox = C::dec(cur.uo[0], dp.invscale[1]);
+            uoy = C::dec(cur.uo[1], dp.invscale[2]);
+            uoz = C::dec(cur.uo[2], dp.invscale[3]);
           }
         }
         R tFx, tFy, tFz;
@@ -273,13 +427,13 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
                          ph.tau_diff * lap + t;
             fst((TA*)a.phiout + zoff, nn, 0, pbA, bA, (TA)po);
           }
-          if (dp.obst && isf == -1) {
+          if (dp.obst && !dp.bb && isf == -1) {
             // LB_int_boundary (boundary:117-212) on the post-moments state
             has_lbc = true;
             const R cssq = (R)(1.0 / 3.0);
-            const R om = relax_rate<TWO, R>(dp, phl, rinv);
             const R pxx = m.xx - m.u * m.u, pyy = m.yy - m.v * m.v, pzz = m.zz - m.w * m.w;
             const R pxy = m.xy - m.u * m.v, pxz = m.xz - m.u * m.w, pyz = m.yz - m.v * m.w;
+            const R om = relax_rate<TWO, R>(dp, phl, rinv, pxx, pyy, pzz, pxy, pxz, pyz);
 #pragma unroll
             for (int q = 0; q < 10; ++q) lbc[q] = 0;
 #pragma unroll
@@ -308,8 +462,38 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       in.xx = m.xx - m.u * m.u; in.yy = m.yy - m.v * m.v; in.zz = m.zz - m.w * m.w;
       in.xy = m.xy - m.u * m.v; in.xz = m.xz - m.u * m.w; in.yz = m.yz - m.v * m.w;
       in.ax = Fx; in.ay = Fy; in.az = Fz;
-      in.om = relax_rate<TWO, R>(dp, phl, rinv);
+      in.om = relax_rate<TWO, R>(dp, phl, rinv, in.xx, in.yy, in.zz, in.xy, in.xz, in.yz);
+      if constexpr (PF) {
+        // every store of this iteration is issued: stream the next plane in
+        lds_wait_all();
+        if (zsrc < zend) {
+          dma_raw<TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, pbT, bT, lds0);
+          if (dp.obst) isf_next = isf_load(zsrc + 1);
+        }
+      }
       collide<LAT, HO, R>(in, f);
+      if (dp.bb && isf == -1 && out_node && own_plane) {
+        // BOUNCE_BACK (fused_LB_kernel_bb, fused:364-382): a population whose
+        // upstream node is solid is replaced by this node's own opposite
+        // post-collision population.  The solid node's pushed population is
+        // exactly zero (its moments and force are zeroed), so the pull below
+        // adds zero and the replacement is this correction.
+        has_lbc = true;
+        const R cssq = (R)(1.0 / 3.0);
+#pragma unroll
+        for (int q = 0; q < 10; ++q) lbc[q] = 0;
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+          const int p = c % 3 - 1, q = (c / 3) % 3 - 1, r = c / 9 - 1;
+          if (c == 13 || !present<LAT>(p, q, r)) continue;
+          if (isf_at(a.isfh, xs - p, ys - q, zsrc - r, nx, ny, nz) != 0) continue;
+          const R fo = f[cidx(-p, -q, -r)];
+          lbc[0] += fo;
+          lbc[1] += fo * (R)p; lbc[2] += fo * (R)q; lbc[3] += fo * (R)r;
+          lbc[4] += fo * ((R)(p * p) - cssq); lbc[5] += fo * ((R)(q * q) - cssq); lbc[6] += fo * ((R)(r * r) - cssq);
+          lbc[7] += fo * (R)(p * q); lbc[8] += fo * (R)(p * r); lbc[9] += fo * (R)(q * r);
+        }
+      }
     }
 
     // ---- 2. x-halo slices through LDS -------------------------------------------
@@ -325,6 +509,9 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
     R G0[9];  // q = 0 partial sums kept in registers: [d][r]
     if (!halo_wave) {
       // ---- 3. x: shift by one lane, contract over p ---------------------------
+      // the 18 x-halo values of this row: lane k < 18 holds sHX[wave][k / 9][k % 9]
+      const R hx = (&sHX[wave][0][0])[lane < 18 ? lane : 17];
+      const bool short_row = XW < 64;  // the right halo enters below lane 63
       R G[3][3][3];  // [d][q][r]
 #pragma unroll
       for (int r = -1; r <= 1; ++r)
@@ -333,14 +520,12 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
           const int qr = (q + 1) + 3 * (r + 1);
           R fm_in = 0, fp_in = 0;
           if (present<LAT>(1, q, r)) {
-            fp_in = shfl_from_lower(f[cidx(1, q, r)]);
-            const R hl = sHX[w][0][qr];
-            if (lane == 0) fp_in = hl;
+            fp_in = shift_from_lower(f[cidx(1, q, r)], lane_value(hx, qr));
           }
           if (present<LAT>(-1, q, r)) {
-            fm_in = shfl_from_upper(f[cidx(-1, q, r)]);
-            const R hr = sHX[w][1][qr];
-            if (lane == XW - 1) fm_in = hr;
+            const R hr = lane_value(hx, 9 + qr);
+            fm_in = shift_from_upper(f[cidx(-1, q, r)], hr);
+            if (short_row) fm_in = lane == XW - 1 ? hr : fm_in;
           }
           const R f0 = present<LAT>(0, q, r) ? f[cidx(0, q, r)] : (R)0;
           const R s = fm_in + f0 + fp_in;
@@ -353,8 +538,8 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          if (w < NW - 1) sUP[w][d * 3 + r][lane] = G[d][2][r];
-          if (w > 0) sDN[w][d * 3 + r][lane] = G[d][0][r];
+          if (wave < NW - 1) sUP[wave][d * 3 + r][lane] = G[d][2][r];
+          if (wave > 0) sDN[wave][d * 3 + r][lane] = G[d][0][r];
           G0[d * 3 + r] = G[d][1][r];
         }
     }
@@ -367,18 +552,18 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         {
-          const R gp = sUP[w - 1][0 * 3 + r][lane], gm = sDN[w + 1][0 * 3 + r][lane], g0 = G0[0 * 3 + r];
+          const R gp = sUP[wave - 1][0 * 3 + r][lane], gm = sDN[wave + 1][0 * 3 + r][lane], g0 = G0[0 * 3 + r];
           H00[r] = gm + g0 + gp;
           H01[r] = gp - gm;
           H02[r] = twothird * (gp + gm) - third * g0;
         }
         {
-          const R gp = sUP[w - 1][1 * 3 + r][lane], gm = sDN[w + 1][1 * 3 + r][lane], g0 = G0[1 * 3 + r];
+          const R gp = sUP[wave - 1][1 * 3 + r][lane], gm = sDN[wave + 1][1 * 3 + r][lane], g0 = G0[1 * 3 + r];
           H10[r] = gm + g0 + gp;
           H11[r] = gp - gm;
         }
         {
-          const R gp = sUP[w - 1][2 * 3 + r][lane], gm = sDN[w + 1][2 * 3 + r][lane], g0 = G0[2 * 3 + r];
+          const R gp = sUP[wave - 1][2 * 3 + r][lane], gm = sDN[wave + 1][2 * 3 + r][lane], g0 = G0[2 * 3 + r];
           H20[r] = gm + g0 + gp;
         }
       }
@@ -399,17 +584,17 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
 #pragma unroll
         for (int q = 0; q < 10; ++q) mid[q] += lbc[q];
       }
-      // ---- 6. plane zsrc-1 is complete --------------------------------------------
-      const int zo = zsrc - 1;
-      if (lane < XW && zo >= z0 && zo < zend) {
-        Mom<R> o;
-        o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
-        o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
-        store_mom_b<PREC>(Sout + (size_t)zo * nxny, nn, pbT, bT, o, dp);
-      }
+      // ---- 6. plane zsrc-1 is complete (stored here, or by the next iteration) --
+      if constexpr (!PF) {
+        const int zo = zsrc - 1;
+        if (lane < XW && zo >= z0 && zo < zend) store_lo(zo);
 #pragma unroll
-      for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+        for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+      }
     }
+  }
+  if constexpr (PF) {
+    if (out_row && lane < XW) store_lo(zend - 1);
   }
 }
 

@@ -32,6 +32,8 @@ struct StageArgs {
   void* frc;         // 3-field set
   void* pre;         // 5-field set of the fast path (phase-field prep output)
   int gf_from_pre;   // PHI_int_boundary: donor gradient from pre instead of aux
+  void* rep;         // REPULSIVE_FLUX: 6-field set, locaux 3..8 (partner index, flux)
+  int8_t* rep_mask;  // REPULSIVE_FLUX: nx*ny*nz
   const int8_t* isfh;
   DevParams dp;
 };

@@ -29,9 +29,11 @@ __device__ __forceinline__ int8_t isf_at(const int8_t* isfh, int x, int y, int z
 // the loaded moments m; returns the stored force tF.  Expression order
 // follows the reference; divisions by rho_phi, the INTERFACE_INCOMP
 // denominator and tau*cs^2 are multiplications by one reciprocal each.
+// rflux: the REPULSIVE_FLUX flux of this node (locaux 6..8), or null.
 template <bool TWO, class R>
 __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx, R gfy, R gfz, R lap, R dvt,
-                                              R uox, R uoy, R uoz, Mom<R>& m, R& tFx, R& tFy, R& tFz) {
+                                              R uox, R uoy, R uoz, Mom<R>& m, R& tFx, R& tFy, R& tFz,
+                                              const R* rflux = nullptr) {
   const Phys<R>& ph = dp.ph<R>();
   const R cssq = (R)(1.0 / 3.0), invcssq = (R)3, HALF = (R)0.5, ONE = (R)1;
   const bool dens = TWO && dp.dens != 0;
@@ -55,6 +57,17 @@ __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx,
   if (TWO) {
     const R st = ((R)4.0 * ph.beta * phl * (phl - (R)1.0) * (phl - (R)0.5) - ph.kapp * lap);
     Fx = Fx + st * gfx; Fy = Fy + st * gfy; Fz = Fz + st * gfz;
+    if (rflux) {
+      // REPULSIVE_FLUX (moments:148-160): rho_phi * flux clamped to +-1e-3, times rho_phi
+      R fr[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        R mt = rflux[c] * rhophi;
+        if (fabs(mt) > (R)1.0e-3) mt = (R)1.0e-3 * (mt < (R)0 ? (R)-1 : (R)1);
+        fr[c] = mt * rhophi;
+      }
+      Fx = Fx + fr[0]; Fy = Fy + fr[1]; Fz = Fz + fr[2];
+    }
   }
   R grx = 0, gry = 0, grz = 0, vt = 0;
   if (dens) {
@@ -107,12 +120,23 @@ __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx,
 }
 
 // relaxation rate of the fused / LB_int_boundary kernels: omega, or
-// 1/(3 nu(phi) + 1/2) with nu(phi) the density-weighted viscosity mix
+// 1/(3 nu(phi) + 1/2) with nu(phi) the density-weighted viscosity mix;
+// under SMAGORINSKI (fused:198-210, 603-616; boundary:145-160)
+//   tau = 1/2 + (3 nu + sqrt((3 nu)^2 + C sqrt(2 Q) / rho_phi)) / 6,
+// Q = Pneq:Pneq, C = 0.053*18.0 evaluated in single precision as the
+// reference's default-real literals are.
 template <bool TWO, class R>
-__device__ __forceinline__ R relax_rate(const DevParams& dp, R phl, R irho) {
+__device__ __forceinline__ R relax_rate(const DevParams& dp, R phl, R irho, R pxx, R pyy, R pzz, R pxy, R pxz,
+                                        R pyz) {
   const Phys<R>& ph = dp.ph<R>();
+  const R visc = TWO ? (ph.rho_r * ph.visc1 * phl + ((R)1 - phl) * ph.visc2 * ph.rho_b) * irho : ph.visc1;
+  if (dp.smag) {
+    const R QQ = pxx * pxx + pyy * pyy + pzz * pzz + (R)2 * (pxy * pxy + pxz * pxz + pyz * pyz);
+    const R C = (R)(0.053f * 18.0f);
+    const R t3 = (R)3 * visc;
+    return rcp((R)0.5 + (R)(1.0 / 6.0) * ((R)3 * visc + sqrt(t3 * t3 + C * sqrt((R)2 * QQ) * irho)));
+  }
   if (!TWO) return ph.omega;
-  const R visc = (ph.rho_r * ph.visc1 * phl + ((R)1 - phl) * ph.visc2 * ph.rho_b) * irho;
   return rcp(visc * (R)3 + (R)0.5);
 }
 

@@ -254,6 +254,144 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 }
 
 // ---------------------------------------------------------------------------
+// Streaming fast-path phase-field prep (float phase-field storage).
+//
+// Same stencils and expression order as k_norm / k_div
+// (lb_cuda_auxfields_module.f90:72-130, 1110-1160), evaluated from LDS.  A
+// block of QW waves owns a tile of 64 lanes in x (x0-2 .. x0+61; outputs on
+// lanes 2..61) by QY + 4 rows of phi and marches z.  phi planes stream into a
+// ring of four LDS planes by LDS DMA three planes ahead; theta planes live in
+// a ring of three.  Iteration k computes grad/lap/theta of plane k and div
+// theta of plane k-1.  Every in-plane index is wrapped once per lane/row,
+// not per element, and isfluid is read only with internal obstacles.
+constexpr int QX = 60, QY = 14, QW = 8;
+constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
+constexpr int QT = QY + 2;                 // theta rows (1-row halo)
+constexpr int QDMA = (QR + QW - 1) / QW;   // phi rows each wave streams per plane
+
+template <int PREC>
+__global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
+  typedef typename Codec<PREC>::R R;
+  typedef typename Aux<PREC>::T TA;
+  static_assert(sizeof(TA) == 4, "float phase-field storage");
+  __shared__ float sphi[4][QR + 1][64];  // row QR absorbs the padding transfers
+  __shared__ R sth[3][3][QT][64];
+
+  const DevParams& dp = a.dp;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ntx = (nx + QX - 1) / QX, nty = (ny + QY - 1) / QY, ntz = (nz + tz - 1) / tz;
+  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int x0 = (tile % ntx) * QX, y0 = ((tile / ntx) % nty) * QY;
+  const int z0 = (tile / (ntx * nty)) * tz, zend = min(z0 + tz, nz);
+  const int xl = x0 - 2 + lane;
+  const int xw = ((xl % nx) + nx) % nx;
+  const bool out_lane = lane >= 2 && lane < 62 && xl < nx;
+  const uint32_t pb = (uint32_t)(nxny * sizeof(TA));
+  const TA* phi = (const TA*)a.phiS;
+  TA* pre = (TA*)a.pre;
+  const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
+  const R THREE = 3, ONE = 1, TWO = 2;
+  const R eps = (R)1.0e-9;
+  const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
+
+  // this wave streams phi rows wave + QW*i; rows past the tile go to the padding row
+  uint32_t rowoff[QDMA], rowlds[QDMA];
+#pragma unroll
+  for (int i = 0; i < QDMA; ++i) {
+    const int r = wave + QW * i;
+    const int yw = (((y0 - 2 + (r < QR ? r : 0)) % ny) + ny) % ny;
+    rowoff[i] = ((uint32_t)yw * (uint32_t)nx + (uint32_t)xw) * (uint32_t)sizeof(TA);
+    rowlds[i] = (uint32_t)(r < QR ? r : QR) * 64u * 4u;
+  }
+  const uint32_t lds_phi = lds_addr(&sphi[0][0][0]);
+  auto dma_plane = [&](int z) {
+    const int zw = ((z % nz) + nz) % nz;
+    const Rsrc rs = mkrsrc(phi + (size_t)zw * nxny, pb);
+    const uint32_t base = lds_phi + (uint32_t)(z & 3) * (QR + 1) * 64u * 4u;
+#pragma unroll
+    for (int i = 0; i < QDMA; ++i) dma_dword(rs, rowoff[i], base + rowlds[i]);
+  };
+  auto fluid = [&](int x, int y, int z) -> bool {
+    if (!dp.obst) return true;
+    const int8_t v = isf_at(a.isfh, x, y, z, nx, ny, nz);
+    return v == 1 || v == -1;
+  };
+
+  dma_plane(z0 - 2);
+  dma_plane(z0 - 1);
+  dma_plane(z0);
+  dma_plane(z0 + 1);
+  for (int k = z0 - 1; k <= zend; ++k) {
+    // phi(k+1) has landed once only the newest plane's QDMA transfers are outstanding
+    if (k + 2 <= zend + 1)
+      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(QDMA) : "memory");
+    else
+      dma_wait_all();
+    __syncthreads();
+
+    // ---- grad / lap / theta of plane k on theta rows t (y = y0 - 1 + t) ------
+    {
+      const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
+      const int tk = (k + 3) % 3;
+      const bool own = k >= z0 && k < zend;
+      for (int t = wave; t < QT; t += QW) {
+        // lanes 0 and 63 read their own column for the missing neighbour;
+        // no output depends on them
+#define F(a_, b_, c_) ((R)sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][t + 1 + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)])
+        const R gx = GRAD_X(F), gy = GRAD_Y(F), gz = GRAD_Z(F);
+        const R c = F(0, 0, 0);
+        const int y = y0 - 1 + t;
+        R thx = 0, thy = 0, thz = 0;
+        if (fluid(xl, y, k)) {
+          const R mg = sqrt(gx * gx + gy * gy + gz * gz);
+          const R im = rcp(mg + eps);
+          const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
+          thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
+          if (own && out_lane && t >= 1 && t <= QY && y < ny) {
+            const R lap = LAPL(F, c);
+            // the reference re-reads n and |g| from storage precision
+            const R mgs = (R)(TA)mg;
+            const size_t o = (size_t)k * nxny + (size_t)y * nx + xl;
+            pre[o] = (TA)((R)(TA)nxv * mgs);
+            pre[o + nn] = (TA)((R)(TA)nyv * mgs);
+            pre[o + 2 * nn] = (TA)((R)(TA)nzv * mgs);
+            pre[o + 3 * nn] = (TA)lap;
+          }
+        }
+#undef F
+        sth[tk][0][t][lane] = (R)(TA)thx;
+        sth[tk][1][t][lane] = (R)(TA)thy;
+        sth[tk][2][t][lane] = (R)(TA)thz;
+      }
+    }
+    __syncthreads();
+
+    // ---- div theta of plane k-1 on theta rows 1..QY -----------------------------
+    const int zo = k - 1;
+    if (zo >= z0 && zo < zend) {
+      const int tm = (k + 1) % 3, t0 = (k + 2) % 3, tp = (k + 3) % 3;  // planes k-2, k-1, k
+      for (int t = 1 + wave; t <= QY; t += QW) {
+#define TH(f_, a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][f_][t + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)]
+#define X(a_, b_, c_) TH(0, a_, b_, c_)
+#define Y(a_, b_, c_) TH(1, a_, b_, c_)
+#define Z(a_, b_, c_) TH(2, a_, b_, c_)
+        const R d = (DX(X) + DY(Y) + DZ(Z)) * THREE;
+#undef X
+#undef Y
+#undef Z
+#undef TH
+        const int y = y0 - 1 + t;
+        if (out_lane && y < ny && fluid(xl, y, zo)) pre[(size_t)zo * nxny + (size_t)y * nx + xl + 4 * nn] = (TA)d;
+      }
+    }
+    // every store of this iteration is issued; slot (k+3)&3 held plane k-1, read above
+    if (k + 3 <= zend + 1) dma_plane(k + 3);
+  }
+}
+
+// ---------------------------------------------------------------------------
 template <int PREC>
 __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   typedef typename Codec<PREC>::R R;
@@ -360,7 +498,13 @@ __global__ void __launch_bounds__(BS) k_moments(StageArgs a) {
       uoy = C::dec(uo[n + nn], dp.invscale[2]);
       uoz = C::dec(uo[n + 2 * nn], dp.invscale[3]);
     }
-    moments_stage<true, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+    R rfl[3] = {0, 0, 0};
+    if (dp.rep) {
+      const TA* rf = (const TA*)a.rep + 3 * nn;
+      rfl[0] = (R)rf[n]; rfl[1] = (R)rf[n + nn]; rfl[2] = (R)rf[n + 2 * nn];
+    }
+    moments_stage<true, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz,
+                           dp.rep ? rfl : nullptr);
   } else {
     moments_stage<false, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
   }
@@ -428,7 +572,14 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
   const R pxx = s.xx - s.u * s.u, pyy = s.yy - s.v * s.v, pzz = s.zz - s.w * s.w;
   const R pxy = s.xy - s.u * s.v, pxz = s.xz - s.u * s.w, pyz = s.yz - s.v * s.w;
   R om;
-  if (dp.twocomp) {
+  if (dp.smag) {
+    // boundary:145-160 (division by rho_phi where fused multiplies by its inverse)
+    const R visc = dp.twocomp ? (ph.rho_r * ph.visc1 * phl + (ONE - phl) * ph.visc2 * ph.rho_b) / rhophi : ph.visc1;
+    const R QQ = pxx * pxx + pyy * pyy + pzz * pzz + TWO * (pxy * pxy + pxz * pxz + pyz * pyz);
+    const R t3 = (R)3.0 * visc;
+    om = (R)0.5 + (R)(1.0 / 6.0) * ((R)3.0 * visc + sqrt(t3 * t3 + (R)(0.053f * 18.0f) * sqrt((R)2.0 * QQ) / rhophi));
+    om = (R)1.0 / om;
+  } else if (dp.twocomp) {
     const R visc = (ph.rho_r * ph.visc1 * phl + (ONE - phl) * ph.visc2 * ph.rho_b) / rhophi;
     om = (visc * invcssq + (R)0.5);
     om = (R)1.0 / om;
@@ -458,6 +609,138 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// REPULSIVE_FLUX (lb_cuda_repulsive_module.f90).  Neighbours up to three
+// nodes away wrap across periodic faces (the reference's 3-deep halo,
+// mpi_module.f90:163-165) and are skipped across a wall face.
+__device__ __forceinline__ bool wrap_win(int& x, int& y, int& z, const DevParams& dp) {
+  auto w = [](int& a, int n, int per) {
+    if (a >= 0 && a < n) return true;
+    if (!per) return false;
+    a = ((a % n) + n) % n;
+    return true;
+  };
+  return w(x, dp.nx, dp.pbc[0]) && w(y, dp.ny, dp.pbc[1]) && w(z, dp.nz, dp.pbc[2]);
+}
+
+// thinfilm_scan_mark_kernel (repulsive:19-150): for an interfacial node, the
+// nearest facing interfacial node in the 7^3 window (ties: larger facing factor).
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_thinfilm_scan(StageArgs a) {
+  typedef typename Codec<PREC>::R R;
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  int x, y, z;
+  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
+  if (isf != 1 && isf != -1) return;
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const Phys<R>& ph = dp.ph<R>();
+  const TA* phi = (const TA*)a.phiS;
+  const TA* aux = (const TA*)a.aux;
+  TA* loc = (TA*)a.rep;
+  a.rep_mask[n] = 0;
+  loc[n] = (TA)0; loc[n + nn] = (TA)0; loc[n + 2 * nn] = (TA)0;
+  R q = fmin(fmax((R)phi[n], (R)0), (R)1);
+  q = q * ((R)1 - q);
+  if (q < ph.q_th) return;
+  const R nix = (R)aux[n], niy = (R)aux[n + nn], niz = (R)aux[n + 2 * nn];
+  R best_r2 = (R)1.0e300, best_face = (R)-1;
+  int bi = 0, bj = 0, bk = 0;
+  bool found = false;
+  for (int di = -3; di <= 3; ++di)
+    for (int dj = -3; dj <= 3; ++dj)
+      for (int dk = -3; dk <= 3; ++dk) {
+        if (di == 0 && dj == 0 && dk == 0) continue;
+        int xi = x + di, yj = y + dj, zk = z + dk;
+        if (!wrap_win(xi, yj, zk, dp)) continue;
+        const int8_t v = isf_at(a.isfh, xi, yj, zk, nx, ny, nz);
+        if (v != 1 && v != -1) continue;
+        const R r2 = (R)(di * di + dj * dj + dk * dk);
+        const size_t m = nidx(xi, yj, zk, nx, ny);
+        R qn = fmin(fmax((R)phi[m], (R)0), (R)1);
+        qn = qn * ((R)1 - qn);
+        if (qn < ph.q_th || fabs(qn - q) > (R)0.1 * fmax(q, (R)1.0e-12)) continue;
+        const R dotn = nix * (R)aux[m] + niy * (R)aux[m + nn] + niz * (R)aux[m + 2 * nn];
+        if (dotn > ph.cos_opp) continue;
+        const R face = (R)0.5 * ((R)1 - dotn);
+        if (r2 < best_r2 - (R)1.0e-14) {
+          best_r2 = r2; best_face = face; bi = x + di; bj = y + dj; bk = z + dk; found = true;
+        } else if (fabs(r2 - best_r2) <= (R)1.0e-14 && face > best_face) {
+          best_face = face; bi = x + di; bj = y + dj; bk = z + dk; found = true;
+        }
+      }
+  if (found) {
+    // the reference stores 1-based local indices
+    loc[n] = (TA)(bi + 1); loc[n + nn] = (TA)(bj + 1); loc[n + 2 * nn] = (TA)(bk + 1);
+    a.rep_mask[n] = 1;
+  }
+}
+
+// repulsive_flux_normal_kernel (repulsive:152-289)
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_repulsive_flux(StageArgs a) {
+  typedef typename Codec<PREC>::R R;
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  int x, y, z;
+  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
+  if (isf != 1 && isf != -1) return;
+  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const Phys<R>& ph = dp.ph<R>();
+  const TA* phi = (const TA*)a.phiS;
+  const TA* aux = (const TA*)a.aux;
+  TA* loc = (TA*)a.rep;
+  TA* flux = loc + 3 * nn;
+  flux[n] = (TA)0; flux[n + nn] = (TA)0; flux[n + 2 * nn] = (TA)0;
+  if (a.rep_mask[n] != 1) return;
+  const R eps = (R)1.0e-9;
+  const R phl = (R)phi[n];
+  const R q1 = phl * ((R)1 - phl);
+  if (q1 <= eps) return;
+  const int pi = (int)(R)loc[n] - 1, pj = (int)(R)loc[n + nn] - 1, pk = (int)(R)loc[n + 2 * nn] - 1;
+  R dx = (R)(pi - x), dy = (R)(pj - y), dz = (R)(pk - z);
+  const R r = sqrt(dx * dx + dy * dy + dz * dz);
+  if (r <= eps) return;
+  const R rinv = (R)1.0 / r;
+  dx = dx * rinv; dy = dy * rinv; dz = dz * rinv;
+  int xi = pi, yj = pj, zk = pk;
+  if (!wrap_win(xi, yj, zk, dp)) return;
+  const size_t m = nidx(xi, yj, zk, nx, ny);
+  const R nx1 = (R)aux[n], ny1 = (R)aux[n + nn], nz1 = (R)aux[n + 2 * nn];
+  const R nx2 = (R)aux[m], ny2 = (R)aux[m + nn], nz2 = (R)aux[m + 2 * nn];
+  const R face = fmax((R)0, -(nx1 * nx2 + ny1 * ny2 + nz1 * nz2));
+  if (face <= eps) return;
+  R nsx = nx1 - nx2, nsy = ny1 - ny2, nsz = nz1 - nz2;
+  const R nsmag = sqrt(nsx * nsx + nsy * nsy + nsz * nsz);
+  if (nsmag <= eps) {
+    nsx = dx; nsy = dy; nsz = dz;
+  } else {
+    nsx = nsx / nsmag; nsy = nsy / nsmag; nsz = nsz / nsmag;
+  }
+  if (dx * nsx + dy * nsy + dz * nsz < (R)0) { nsx = -nsx; nsy = -nsy; nsz = -nsz; }
+  const R phl2 = (R)phi[m];
+  const R q2 = phl2 * ((R)1 - phl2);
+  const R qpair = (R)0.5 * (q1 + q2);
+  const R qcl = fmin(fmax(qpair, eps), (R)0.25 - eps);
+  const R arg = (R)1.0 / ((R)2.0 * sqrt(qcl));
+  if (arg <= (R)1.0) return;
+  const R ach = log(arg + sqrt(arg * arg - (R)1.0));
+  const R Wfilm = ph.width * ach;
+  if (Wfilm <= (R)0) return;
+  const R w2 = Wfilm * Wfilm;
+  const R wdth = (R)1.0 / ((R)1.0 + w2 * w2);
+  const R fx = ph.a_rep * wdth * qcl * face * nsx;
+  const R fy = ph.a_rep * wdth * qcl * face * nsy;
+  const R fz = ph.a_rep * wdth * qcl * face * nsz;
+  const R cap = (R)1.5 * (fabs(fx) + fabs(fy) + fabs(fz));
+  const R scales = fmin((R)1.0, phl / fmax(cap, (R)1.0e-9));
+  flux[n] = (TA)(fx * scales); flux[n + nn] = (TA)(fy * scales); flux[n + 2 * nn] = (TA)(fz * scales);
+}
+
+// ---------------------------------------------------------------------------
 template <int PREC>
 static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
   const size_t n = a.dp.nn;
@@ -469,6 +752,8 @@ static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
     case 4: hipLaunchKernelGGL(k_moments<PREC>, grid, block, 0, s, a); break;
     case 6: hipLaunchKernelGGL(k_update_phi<PREC>, grid, block, 0, s, a); break;
     case 7: hipLaunchKernelGGL(k_lb_int_bnd<PREC>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_thinfilm_scan<PREC>, grid, block, 0, s, a); break;
+    case 9: hipLaunchKernelGGL(k_repulsive_flux<PREC>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -487,8 +772,13 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
 
 template <int PREC>
 static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
-  const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
-  hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
+  if constexpr (sizeof(typename Aux<PREC>::T) == 4) {
+    const dim3 grid(((a.dp.nx + QX - 1) / QX) * ((a.dp.ny + QY - 1) / QY) * ((a.dp.nz + tz - 1) / tz));
+    hipLaunchKernelGGL(k_phase_prep_z<PREC>, grid, dim3(QW * 64), 0, s, a, tz);
+  } else {
+    const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
+    hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
+  }
   return hipGetLastError();
 }
 

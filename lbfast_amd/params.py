@@ -25,14 +25,20 @@ FIELD_LAPPHI, FIELD_DIVTHN = 25, 26
 FIELD_O = tuple(range(27, 37))
 FIELD_PHI_O = 37
 FIELD_RHOPHI = 40
+FIELD_REP_PARTNER = (41, 42, 43)  # locaux 3..5
+FIELD_REP_FLUX = (44, 45, 46)     # locaux 6..8
+FIELD_REP_MASK = 47
 HFIELDS = tuple(range(1, 11))
 HFIELD_NAMES = ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
 
 STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N = 1, 2, 3
 STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY = 4, 5, 6, 7
+STAGE_THINFILM_SCAN, STAGE_REPULSIVE_FLUX = 8, 9
 # integrator_module.f90:278-524 (one FLIP half step)
-STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N, STAGE_MOMENTS,
-              STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY)
+# integrator_module.f90:266-545; the repulsive stages and LB_int_boundary are
+# no-ops unless REPULSIVE_FLUX / INTERNAL_OBSTACLES without BOUNCE_BACK
+STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N, STAGE_THINFILM_SCAN,
+              STAGE_REPULSIVE_FLUX, STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY)
 
 
 class LbfParams(ctypes.Structure):
@@ -42,7 +48,8 @@ class LbfParams(ctypes.Structure):
         ("twocomponent", ctypes.c_int32), ("densratio", ctypes.c_int32),
         ("interface_incomp", ctypes.c_int32), ("internal_obstacles", ctypes.c_int32),
         ("buoyancy_forcing", ctypes.c_int32), ("pbc", ctypes.c_int32 * 3),
-        ("precision", ctypes.c_int32), ("reserved_i", ctypes.c_int32 * 4),
+        ("precision", ctypes.c_int32), ("bounce_back", ctypes.c_int32),
+        ("smagorinsky", ctypes.c_int32), ("repulsive_flux", ctypes.c_int32), ("reserved_i", ctypes.c_int32),
         ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
         ("visc1", ctypes.c_double), ("visc2", ctypes.c_double),
         ("rho_r", ctypes.c_double), ("rho_b", ctypes.c_double),
@@ -50,7 +57,9 @@ class LbfParams(ctypes.Structure):
         ("wettab_r", ctypes.c_double),
         ("tau1", ctypes.c_double), ("omega", ctypes.c_double), ("tau2", ctypes.c_double),
         ("beta", ctypes.c_double), ("kapp", ctypes.c_double), ("sharp_c", ctypes.c_double),
-        ("store_scale", ctypes.c_double * 10), ("reserved_d", ctypes.c_double * 4),
+        ("store_scale", ctypes.c_double * 10),
+        ("a_rep", ctypes.c_double), ("q_th", ctypes.c_double), ("cos_opp_t", ctypes.c_double),
+        ("reserved_d", ctypes.c_double),
     ]
 
     @property
@@ -89,7 +98,8 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
                 densratio=False, internal_obstacles=False, pbc=(1, 1, 1), precision=PREC_FP32,
                 fx=0.0, fy=0.0, fz=0.0, visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=1.0,
                 tau_diff=0.1, sigma=0.0, width=4.0, wettab_r=90.0,
-                interface_incomp=None, buoyancy_forcing=False) -> LbfParams:
+                interface_incomp=None, buoyancy_forcing=False, bounce_back=False, smagorinsky=False,
+                repulsive_flux=False, a_rep=0.0, q_th=0.125, cos_opp_t=-0.8) -> LbfParams:
     """Build a parameter block. ``interface_incomp`` defaults to ``densratio``
     because defines.h:72-74 defines INTERFACE_INCOMP whenever DENSRATIO is set."""
     p = LbfParams()
@@ -111,6 +121,10 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
     p.rho_r, p.rho_b = rho_r, rho_b
     p.tau_diff, p.sigma, p.width = tau_diff, sigma, width
     p.wettab_r = wettab_r
+    p.bounce_back = int(bool(bounce_back))
+    p.smagorinsky = int(bool(smagorinsky))
+    p.repulsive_flux = int(bool(repulsive_flux) and p.twocomponent)
+    p.a_rep, p.q_th, p.cos_opp_t = a_rep, q_th, cos_opp_t
     return derive(p)
 
 
@@ -153,7 +167,8 @@ def _conv(s: str):
 
 def params_from_namelist(text: str, lattice=27, highorder=True, twocomponent=True,
                          densratio=True, internal_obstacles=False,
-                         precision=PREC_FP32) -> LbfParams:
+                         precision=PREC_FP32, bounce_back=False, smagorinsky=False,
+                         repulsive_flux=False) -> LbfParams:
     """Parameter block from a reference input deck plus defines.h switches."""
     nml = read_namelists(text)
     sim, fl = nml.get("simulation", {}), nml.get("fluid", {})
@@ -166,4 +181,6 @@ def params_from_namelist(text: str, lattice=27, highorder=True, twocomponent=Tru
         visc1=fl.get("visc1", 0.1), visc2=fl.get("visc2", fl.get("visc1", 0.1)),
         rho_r=fl.get("rho_r", 1.0), rho_b=fl.get("rho_b", 1.0),
         tau_diff=fl.get("tau_diff", 0.1), sigma=fl.get("sigma", 0.0), width=fl.get("width", 4.0),
-        wettab_r=fl.get("wettab_r", 90.0))
+        wettab_r=fl.get("wettab_r", 90.0), bounce_back=bounce_back, smagorinsky=smagorinsky,
+        repulsive_flux=repulsive_flux, a_rep=fl.get("a_rep", 0.0), q_th=fl.get("q_th", 0.125),
+        cos_opp_t=fl.get("cosoppt", -0.8))

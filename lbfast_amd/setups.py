@@ -128,6 +128,27 @@ def random_spheres(shape, fill=0.25, rmin=6.0, rmax=14.0, seed=42, wall_axis=1, 
     return isf
 
 
+def thin_film(shape, thickness, width, axis=0, noise=0.0, seed=5):
+    """A slab of phase phi = 1 between two flat tanh interfaces a distance
+    ``thickness`` apart, centred in the box along ``axis`` (facing interfaces
+    for REPULSIVE_FLUX); P = 0, velocity uniform noise U(-noise, noise)."""
+    gi, gj, gk = _grid(shape)
+    coord = (gi, gj, gk)[axis]
+    L = shape[axis]
+    xc = 0.5 * (L + 1)
+    x1, x2 = xc - 0.5 * thickness, xc + 0.5 * thickness
+    phi = 0.5 * (np.tanh(2.0 * (coord - x1) / width) - np.tanh(2.0 * (coord - x2) / width))
+    z = np.zeros_like(phi)
+    if noise > 0:
+        rng = np.random.default_rng(seed)
+        u, v, w = (rng.uniform(-noise, noise, phi.shape) for _ in range(3))
+    else:
+        u, v, w = z.copy(), z.copy(), z.copy()
+    out = _moments(z.copy(), u, v, w)
+    out["phi"] = phi
+    return out
+
+
 def two_layer_channel(shape, isfluid, frac=0.4, width=4.0, rho_r=10.0, rho_b=1.0, axis=1):
     """Dense phase (phi = 1) in the lower ``frac`` of the channel height along
     ``axis`` with a tanh interface; fluid at rest; P = 0."""

@@ -19,6 +19,11 @@
 //   fused stream+collide    lb_cuda_fused_module.f90:426-750
 //   update_phi              lb_cuda_update_phi_module.f90:19-104
 //   LB_int_boundary         lb_cuda_boundary_module.f90:19-216
+//   BOUNCE_BACK fused       lb_cuda_fused_module.f90:20-424 (fused_LB_kernel_bb)
+//   SMAGORINSKI relaxation  lb_cuda_fused_module.f90:198-210, 603-616;
+//                           lb_cuda_boundary_module.f90:145-160
+//   REPULSIVE_FLUX          lb_cuda_repulsive_module.f90:19-292 and
+//                           lb_cuda_moments_module.f90:148-160
 //   halo (intpbc) copies    mpi_module.f90:646-709, 2820-2918
 //   stage order             integrator_module.f90:179-233, 266-545
 //   initial O := S          init_conditions_module.f90:584-587
@@ -40,6 +45,7 @@
 #include <string>
 #include <algorithm>
 #include <chrono>
+#include <limits>
 #include "../include/lbfast_hip.h"
 
 namespace orc {
@@ -199,12 +205,12 @@ struct Sim : SimBase {
   size_t N;
   std::vector<T> h[2][10];
   std::vector<T> phi[2];
-  std::vector<T> aux[7], loc[2], frc[3];
-  std::vector<int8_t> isf;
+  std::vector<T> aux[7], loc[8], frc[3];  // loc = locaux 1..8
+  std::vector<int8_t> isf, rep_mask;
   std::vector<R> pop;  // post-collision populations of the last fused stage
   int cur = 0;         // index of the S buffer
   int stepno = 0;
-  bool two, dens, incomp, obst, ho;
+  bool two, dens, incomp, obst, ho, bb, smag, rep;
   R Minv[27][27];
 
   explicit Sim(const lbf_params& pp) : p(pp) {
@@ -221,11 +227,15 @@ struct Sim : SimBase {
     for (auto& a : loc) a.assign(N, T(0));
     for (auto& a : frc) a.assign(N, T(0));
     isf.assign(N, 1);
+    rep_mask.assign(N, 0);
     two = p.twocomponent != 0;
     dens = two && p.densratio != 0;
     incomp = dens && p.interface_incomp != 0;
     obst = p.internal_obstacles != 0;
     ho = p.highorder != 0 && p.lattice == 27;
+    bb = obst && p.bounce_back != 0;
+    smag = p.smagorinsky != 0;
+    rep = two && p.repulsive_flux != 0;
     for (int k = 0; k < 27; ++k)
       for (int l = 0; l < 27; ++l) Minv[k][l] = (R)minv_d3q27(k, l);
   }
@@ -233,6 +243,18 @@ struct Sim : SimBase {
   inline size_t I(int i, int j, int k) const { return (size_t)i + (size_t)hx * (j + (size_t)hy * k); }
   static inline R ld(const std::vector<T>& a, size_t n) { return (R)a[n]; }
   static inline T st(R v) { return (T)v; }
+
+  // SMAGORINSKI relaxation rate (fused:603-616): tau = 1/2 + (3 nu +
+  // sqrt((3 nu)^2 + C sqrt(2 Q) / rho_phi)) / 6 with Q = Pneq:Pneq.  The
+  // reference writes C as 0.053*18.0, a product of two default (single
+  // precision) reals, so C is that float32 product widened to double.
+  static R smag_omega(R visc, R pxx, R pyy, R pzz, R pxy, R pxz, R pyz, R inv_rho) {
+    const R QQ = pxx * pxx + pyy * pyy + pzz * pzz + (R)2 * (pxy * pxy + pxz * pxz + pyz * pyz);
+    const R C = (R)(double)(0.053f * 18.0f);
+    const R t3 = (R)3.0 * visc;
+    R tau = (R)0.5 + ((R)1.0 / (R)6.0) * ((R)3.0 * visc + std::sqrt(t3 * t3 + C * std::sqrt((R)2.0 * QQ) * inv_rho));
+    return (R)1.0 / tau;
+  }
 
   // ---- halo (intpbc) ----------------------------------------------------
   // A halo node is copied from its periodic image only when every axis on
@@ -429,6 +451,16 @@ struct Sim : SimBase {
             Fx = Fx + st_ * gfx;
             Fy = Fy + st_ * gfy;
             Fz = Fz + st_ * gfz;
+            if (rep) {
+              // moments:148-160: clamp rho_phi * flux to +-1e-3, add it times rho_phi
+              R fr[3];
+              for (int c = 0; c < 3; ++c) {
+                R mt = ld(loc[5 + c], n) * rhophi;
+                if (std::abs(mt) > (R)1.0e-3) mt = (R)1.0e-3 * (mt < 0 ? (R)-1 : (R)1);
+                fr[c] = mt * rhophi;
+              }
+              Fx = Fx + fr[0]; Fy = Fy + fr[1]; Fz = Fz + fr[2];
+            }
           }
           R grx = 0, gry = 0, grz = 0;
           if (dens) {
@@ -511,8 +543,10 @@ struct Sim : SimBase {
     pxx = pxx - u * u; pyy = pyy - v * v; pzz = pzz - w * w;
     pxy = pxy - u * v; pxz = pxz - u * w; pyz = pyz - v * w;
     R om;
-    if (two) {
-      R visc = ((R)p.rho_r * (R)p.visc1 * phl + (ONE - phl) * (R)p.visc2 * (R)p.rho_b) * inv;
+    const R visc = two ? ((R)p.rho_r * (R)p.visc1 * phl + (ONE - phl) * (R)p.visc2 * (R)p.rho_b) * inv : (R)p.visc1;
+    if (smag) {
+      om = smag_omega(visc, pxx, pyy, pzz, pxy, pxz, pyz, inv);
+    } else if (two) {
       om = (visc * invcssq + (R)0.5);
       om = (R)1.0 / om;
     } else {
@@ -595,7 +629,12 @@ struct Sim : SimBase {
           ozz = ozz + f0 * (-cssq);
           for (int l = 1; l < nl; ++l) {
             const R ex = (R)EX[l], ey = (R)EY[l], ez = (R)EZ[l];
-            R f = pop[I(i - EX[l], j - EY[l], k - EZ[l]) * 27 + l];
+            const size_t up = I(i - EX[l], j - EY[l], k - EZ[l]);
+            R f = pop[up * 27 + l];
+            // BOUNCE_BACK (fused:364-382): the solid upstream node hands the
+            // node's own opposite population back (fluid nodes; solid-node
+            // outputs are unused and order-dependent in the reference)
+            if (bb && isf[n] != 0 && isf[up] == 0) f = pop[n * 27 + OPP[l]];
             op = op + f;
             ou = ou + f * ex; ov = ov + f * ey; ow = ow + f * ez;
             oxx = oxx + f * (ex * ex - cssq);
@@ -649,8 +688,16 @@ struct Sim : SimBase {
           pxx = pxx - u * u; pyy = pyy - v * v; pzz = pzz - w * w;
           pxy = pxy - u * v; pxz = pxz - u * w; pyz = pyz - v * w;
           R om;
-          if (two) {
-            R visc = ((R)p.rho_r * (R)p.visc1 * phl + (ONE - phl) * (R)p.visc2 * (R)p.rho_b) / rhophi;
+          const R visc = two ? ((R)p.rho_r * (R)p.visc1 * phl + (ONE - phl) * (R)p.visc2 * (R)p.rho_b) / rhophi
+                             : (R)p.visc1;
+          if (smag) {
+            // boundary:150-155 divides by rho_phi where fused multiplies by its inverse
+            const R QQ = pxx * pxx + pyy * pyy + pzz * pzz + TWO * (pxy * pxy + pxz * pxz + pyz * pyz);
+            const R C = (R)(double)(0.053f * 18.0f);
+            const R t3 = (R)3.0 * visc;
+            om = (R)0.5 + ((R)1.0 / (R)6.0) * ((R)3.0 * visc + std::sqrt(t3 * t3 + C * std::sqrt((R)2.0 * QQ) / rhophi));
+            om = (R)1.0 / om;
+          } else if (two) {
             om = (visc * invcssq + (R)0.5);
             om = (R)1.0 / om;
           } else {
@@ -678,6 +725,122 @@ struct Sim : SimBase {
         }
   }
 
+  // ---- REPULSIVE_FLUX (lb_cuda_repulsive_module.f90) --------------------------------
+  // Neighbours up to 3 nodes away wrap periodically (the reference's 3-deep
+  // halo copies, mpi_module.f90:163-165); across a non-periodic face they are
+  // skipped.
+  bool wrap3(int& i, int& j, int& k) const {
+    auto w = [](int& a, int n, int per) {
+      if (a >= 1 && a <= n) return true;
+      if (!per) return false;
+      a = ((a - 1) % n + n) % n + 1;
+      return true;
+    };
+    return w(i, nx, p.pbc[0]) && w(j, ny, p.pbc[1]) && w(k, nz, p.pbc[2]);
+  }
+
+  // thinfilm_scan_mark_kernel (repulsive:19-150)
+  void thinfilm_scan(const std::vector<T>& ph) {
+    const int win = 3;  // LBFAST.f90:221
+    const R q_th = (R)p.q_th, cosOpp = (R)p.cos_opp_t;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          size_t n = I(i, j, k);
+          if (std::abs((int)isf[n]) != 1) continue;
+          rep_mask[n] = 0;
+          loc[2][n] = T(0); loc[3][n] = T(0); loc[4][n] = T(0);
+          R q = std::min(std::max(ld(ph, n), (R)0), (R)1);
+          q = q * ((R)1 - q);
+          if (q < q_th) continue;
+          const R nix = ld(aux[0], n), niy = ld(aux[1], n), niz = ld(aux[2], n);
+          R best_r2 = std::numeric_limits<R>::max(), best_face = (R)-1;
+          int bi = 0, bj = 0, bk = 0;
+          bool found = false;
+          for (int di = -win; di <= win; ++di)
+            for (int dj = -win; dj <= win; ++dj)
+              for (int dk = -win; dk <= win; ++dk) {
+                if (di == 0 && dj == 0 && dk == 0) continue;
+                int a = i + di, b = j + dj, c = k + dk;
+                if (!wrap3(a, b, c)) continue;
+                const size_t m = I(a, b, c);
+                if (std::abs((int)isf[m]) != 1) continue;
+                const R r2 = (R)di * (R)di + (R)dj * (R)dj + (R)dk * (R)dk;
+                if (r2 < (R)1.0e-12) continue;
+                R qn = std::min(std::max(ld(ph, m), (R)0), (R)1);
+                qn = qn * ((R)1 - qn);
+                if (qn < q_th || std::abs(qn - q) > (R)0.1 * std::max(q, (R)1.0e-12)) continue;
+                const R dotn = nix * ld(aux[0], m) + niy * ld(aux[1], m) + niz * ld(aux[2], m);
+                if (dotn > cosOpp) continue;
+                const R face = (R)0.5 * ((R)1 - dotn);
+                if (r2 < best_r2 - (R)1.0e-14) {
+                  best_r2 = r2; best_face = face; bi = i + di; bj = j + dj; bk = k + dk; found = true;
+                } else if (std::abs(r2 - best_r2) <= (R)1.0e-14) {
+                  if (face > best_face) { best_face = face; bi = i + di; bj = j + dj; bk = k + dk; found = true; }
+                }
+              }
+          if (found) {
+            loc[2][n] = (T)(R)bi; loc[3][n] = (T)(R)bj; loc[4][n] = (T)(R)bk;
+            rep_mask[n] = 1;
+          }
+        }
+  }
+
+  // repulsive_flux_normal_kernel (repulsive:152-289)
+  void repulsive_flux(const std::vector<T>& ph) {
+    const R eps = (R)1.0e-9, width = (R)p.width, A_rep = (R)p.a_rep;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          size_t n = I(i, j, k);
+          if (std::abs((int)isf[n]) != 1) continue;
+          loc[5][n] = T(0); loc[6][n] = T(0); loc[7][n] = T(0);
+          if (rep_mask[n] != 1) continue;
+          const R phl = ld(ph, n);
+          const R q1 = phl * ((R)1 - phl);
+          if (q1 <= eps) continue;
+          const int pi = (int)ld(loc[2], n), pj = (int)ld(loc[3], n), pk = (int)ld(loc[4], n);
+          R dx = (R)(pi - i), dy = (R)(pj - j), dz = (R)(pk - k);
+          const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+          if (r <= eps) continue;
+          const R rinv = (R)1.0 / r;
+          dx = dx * rinv; dy = dy * rinv; dz = dz * rinv;
+          int a = pi, b = pj, c = pk;
+          if (!wrap3(a, b, c)) continue;
+          const size_t m = I(a, b, c);
+          const R nx1 = ld(aux[0], n), ny1 = ld(aux[1], n), nz1 = ld(aux[2], n);
+          const R nx2 = ld(aux[0], m), ny2 = ld(aux[1], m), nz2 = ld(aux[2], m);
+          const R face = std::max((R)0, -(nx1 * nx2 + ny1 * ny2 + nz1 * nz2));
+          if (face <= eps) continue;
+          R nsx = nx1 - nx2, nsy = ny1 - ny2, nsz = nz1 - nz2;
+          const R nsmag = std::sqrt(nsx * nsx + nsy * nsy + nsz * nsz);
+          if (nsmag <= eps) {
+            nsx = dx; nsy = dy; nsz = dz;
+          } else {
+            nsx = nsx / nsmag; nsy = nsy / nsmag; nsz = nsz / nsmag;
+          }
+          if (dx * nsx + dy * nsy + dz * nsz < (R)0) { nsx = -nsx; nsy = -nsy; nsz = -nsz; }
+          const R phl2 = ld(ph, m);
+          const R q2 = phl2 * ((R)1 - phl2);
+          const R qpair = (R)0.5 * (q1 + q2);
+          const R qcl = std::min(std::max(qpair, eps), (R)0.25 - eps);
+          const R arg = (R)1.0 / ((R)2.0 * std::sqrt(qcl));
+          if (arg <= (R)1.0) continue;
+          const R ach = std::log(arg + std::sqrt(arg * arg - (R)1.0));
+          const R Wfilm = width * ach;
+          if (Wfilm <= (R)0) continue;
+          const R wdth = (R)1.0 / ((R)1.0 + std::pow(Wfilm, (R)4.0));
+          const R fx = A_rep * wdth * qcl * face * nsx;
+          const R fy = A_rep * wdth * qcl * face * nsy;
+          const R fz = A_rep * wdth * qcl * face * nsz;
+          const R cap = (R)1.5 * (std::abs(fx) + std::abs(fy) + std::abs(fz));
+          const R scales = std::min((R)1.0, phl / std::max(cap, (R)1.0e-9));
+          loc[5][n] = st(fx * scales); loc[6][n] = st(fy * scales); loc[7][n] = st(fz * scales);
+        }
+  }
+
   // ---- driver-level entry points ---------------------------------------------------
   int begin_step() override { ++stepno; return 0; }
   int end_step() override { cur = 1 - cur; return 0; }
@@ -694,7 +857,9 @@ struct Sim : SimBase {
       case LBF_STAGE_MOMENTS: moments(O, S, pS); return 0;
       case LBF_STAGE_FUSED: fused(S, O, pS); return 0;
       case LBF_STAGE_UPDATE_PHI: if (!two) return 0; update_phi(S, pS, pO); return 0;
-      case LBF_STAGE_LB_INT_BOUNDARY: if (!obst) return 0; lb_int_boundary(S, O, pS); return 0;
+      case LBF_STAGE_LB_INT_BOUNDARY: if (!obst || bb) return 0; lb_int_boundary(S, O, pS); return 0;
+      case LBF_STAGE_THINFILM_SCAN: if (!rep) return 0; thinfilm_scan(pS); return 0;
+      case LBF_STAGE_REPULSIVE_FLUX: if (!rep) return 0; repulsive_flux(pS); return 0;
       default: err = "unknown stage"; return LBF_EINVAL;
     }
   }
@@ -706,6 +871,8 @@ struct Sim : SimBase {
       stage(LBF_STAGE_NORM);
       stage(LBF_STAGE_PHI_INT_BOUNDARY);
       stage(LBF_STAGE_DIV_THETA_N);
+      stage(LBF_STAGE_THINFILM_SCAN);
+      stage(LBF_STAGE_REPULSIVE_FLUX);
       stage(LBF_STAGE_MOMENTS);
       stage(LBF_STAGE_FUSED);
       stage(LBF_STAGE_UPDATE_PHI);
@@ -738,6 +905,7 @@ struct Sim : SimBase {
     if (id >= 15 && id <= 17) return &frc[id - 15];
     if (id >= 18 && id <= 24) return &aux[id - 18];
     if (id == 25 || id == 26) return &loc[id - 25];
+    if (id >= 41 && id <= 46) return &loc[id - 39];
     if (id >= 27 && id <= 36) return &h[1 - cur][id - 27];
     if (id == LBF_FIELD_PHI_O) return &phi[1 - cur];
     return nullptr;
@@ -754,6 +922,13 @@ struct Sim : SimBase {
     return 0;
   }
   int get_field(int id, double* dst) override {
+    if (id == LBF_FIELD_REP_MASK) {
+      for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+          for (int i = 1; i <= nx; ++i)
+            dst[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))] = (double)rep_mask[I(i, j, k)];
+      return 0;
+    }
     if (id == LBF_FIELD_RHOPHI) {
       for (int k = 1; k <= nz; ++k)
         for (int j = 1; j <= ny; ++j)

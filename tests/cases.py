@@ -9,7 +9,18 @@ from lbfast_amd.params import (make_params, PREC_FP64, PREC_FP32, PREC_MIXED, HF
 
 
 def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
-    """(params, fields, raw isfluid or None) for a named test workload."""
+    """(params, fields, raw isfluid or None) for a named test workload.
+    Suffixes: ``_bb`` BOUNCE_BACK, ``_smag`` SMAGORINSKI."""
+    for suffix, key in (("_bb", "bounce_back"), ("_smag", "smagorinsky")):
+        if name.endswith(suffix):
+            p, f, raw = case(name[: -len(suffix)], n, precision, seed)
+            setattr(p, key, 1)
+            return p, f, raw
+    if name == "film27h_rep":  # two facing interfaces 4 nodes apart, REPULSIVE_FLUX
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True,
+                        visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=0.1, tau_diff=0.1, sigma=0.01, width=2.0,
+                        precision=precision, repulsive_flux=True, a_rep=0.05, q_th=0.125, cos_opp_t=-0.8)
+        return p, setups.thin_film(n, 4.0, 2.0, axis=0, noise=1e-5, seed=seed), None
     if name == "tg19":  # Taylor-Green, D3Q19, one component
         p = make_params(n, lattice=19, highorder=False, visc1=(0.8 - 0.5) / 3, precision=precision)
         return p, setups.taylor_green(n, 0.02), None

@@ -121,6 +121,32 @@ def test_stage_path(name, prec):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("name", ["chan27h_bb", "chan19_bb", "pois27h_1c_bb"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+@pytest.mark.parametrize("path", ["fast", "staged"])
+def test_bounce_back(name, prec, path):
+    """BOUNCE_BACK: fused_LB_kernel_bb's reflection at solid nodes, no LB_int_boundary."""
+    res, bad = run_pair(name, prec, (40, 24, 32), 8, path=path)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["tg27h_smag", "drop27h_smag", "chan19_smag"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+@pytest.mark.parametrize("path", ["fast", "staged"])
+def test_smagorinsky(name, prec, path):
+    """SMAGORINSKI relaxation in the fused and LB_int_boundary kernels."""
+    res, bad = run_pair(name, prec, (32, 24, 32), 8, path=path)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_repulsive_flux(prec):
+    """REPULSIVE_FLUX: facing-partner scan, repulsive flux and its force (the
+    step runs the stage sequence); also the partner indices and fluxes."""
+    res, bad = run_pair("film27h_rep", prec, (24, 16, 16), 6)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("n", [(64, 16, 16), (96, 20, 10), (16, 40, 12), (130, 9, 7)])
 def test_tiling_edges(n):
     """Partial x tiles (nx not a multiple of 64), ny not a multiple of the
