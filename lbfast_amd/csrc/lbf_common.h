@@ -102,23 +102,25 @@ __device__ __forceinline__ void fst(T* plane0, size_t fs, int f, uint32_t pbytes
 // cost.  Issued from inline asm so that the compiler's wait-count tracking
 // (which would otherwise make every later s_barrier wait for the transfer)
 // does not see it; the consumer waits with dma_wait_all() before reading.
-// M0 is saved and restored around the load, and s_nop 4 covers the
-// M0-write and SGPR-write hazards ahead of the memory instruction.
+// M0 (= lds + OFF, OFF a compile-time byte offset) is saved and restored
+// around the load, and s_nop 4 covers the M0-write and SGPR-write hazards
+// ahead of the memory instruction.
 typedef __attribute__((address_space(3))) float lds_float;
 __device__ __forceinline__ uint32_t lds_addr(float* p) {
   return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_float*)p);
 }
+template <uint32_t OFF = 0>
 __device__ __forceinline__ void dma_dword(Rsrc r, uint32_t voff, uint32_t lds) {
   uint32_t save;
   asm volatile(
       "s_mov_b32 %0, m0\n\t"
-      "s_mov_b32 m0, %3\n\t"
+      "s_add_u32 m0, %3, %4\n\t"
       "s_nop 4\n\t"
       "buffer_load_dword %1, %2, 0 offen lds\n\t"
       "s_mov_b32 m0, %0"
       : "=&s"(save)
-      : "v"(voff), "s"(r), "s"(lds)
-      : "memory");
+      : "v"(voff), "s"(r), "s"(lds), "n"(OFF)
+      : "memory", "scc");
 }
 // Wait for every outstanding vector-memory operation of this wave (the
 // counter also covers stores, so issue a plane's stores before its DMA).

@@ -158,15 +158,6 @@ __device__ __forceinline__ double shift_from_upper(double v, double fill) {
   const int2 t = __builtin_bit_cast(int2, v), u = __builtin_bit_cast(int2, fill);
   return __builtin_bit_cast(double, make_int2(dpp_shl1(u.x, t.x), dpp_shl1(u.y, t.y)));
 }
-// value of lane l (wave-uniform result)
-__device__ __forceinline__ float lane_value(float v, int l) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ __forceinline__ double lane_value(double v, int l) {
-  const int2 t = __builtin_bit_cast(int2, v);
-  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(t.x, l), __builtin_amdgcn_readlane(t.y, l)));
-}
-
 // Encoded inputs of one node for one source plane.
 template <int PREC>
 struct RawNode {
@@ -186,12 +177,11 @@ struct Slots {
 
 // Direct loads of one plane's inputs (used when the inputs are not prefetched).
 template <int PREC, bool TWO, bool INLINE>
-__device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, size_t zoff, uint32_t pbT,
+__device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, size_t zoff, size_t nn, uint32_t pbT,
                                          uint32_t pbA, uint32_t bT, uint32_t bA) {
   typedef typename Codec<PREC>::T T;
   typedef typename Aux<PREC>::T TA;
   const DevParams& dp = a.dp;
-  const size_t nn = dp.nn;
   const T* S = (const T*)a.min + zoff;
 #pragma unroll
   for (int f = 0; f < 10; ++f) r.m[f] = fld(S, nn, f, pbT, bT);
@@ -211,26 +201,40 @@ __device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, s
 
 // LDS-DMA prefetch of one plane's inputs into this wave's slots (FP32 storage).
 template <bool TWO, bool INLINE>
-__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, uint32_t pb, uint32_t boff, uint32_t lds0) {
+__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, size_t nn, uint32_t pb, uint32_t boff,
+                                        uint32_t lds0) {
   typedef Slots<TWO, INLINE> SL;
   const DevParams& dp = a.dp;
-  const size_t nn = dp.nn;
   const float* S = (const float*)a.min + zoff;
-#pragma unroll
-  for (int f = 0; f < 10; ++f) dma_dword(mkrsrc(S + (size_t)f * nn, pb), boff, lds0 + 256u * f);
-  if constexpr (TWO) dma_dword(mkrsrc((const float*)a.phin + zoff, pb), boff, lds0 + 256u * SL::PHI);
+  // slot offsets are immediates: only the wave's LDS base stays live
+  dma_dword<256u * 0>(mkrsrc(S + 0 * nn, pb), boff, lds0);
+  dma_dword<256u * 1>(mkrsrc(S + 1 * nn, pb), boff, lds0);
+  dma_dword<256u * 2>(mkrsrc(S + 2 * nn, pb), boff, lds0);
+  dma_dword<256u * 3>(mkrsrc(S + 3 * nn, pb), boff, lds0);
+  dma_dword<256u * 4>(mkrsrc(S + 4 * nn, pb), boff, lds0);
+  dma_dword<256u * 5>(mkrsrc(S + 5 * nn, pb), boff, lds0);
+  dma_dword<256u * 6>(mkrsrc(S + 6 * nn, pb), boff, lds0);
+  dma_dword<256u * 7>(mkrsrc(S + 7 * nn, pb), boff, lds0);
+  dma_dword<256u * 8>(mkrsrc(S + 8 * nn, pb), boff, lds0);
+  dma_dword<256u * 9>(mkrsrc(S + 9 * nn, pb), boff, lds0);
+  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + zoff, pb), boff, lds0);
   if constexpr (!INLINE) {
     const float* fr = (const float*)a.frc + zoff;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) dma_dword(mkrsrc(fr + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::G + f));
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(fr + 1 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * nn, pb), boff, lds0);
   } else if constexpr (TWO) {
     const float* pr = (const float*)a.pre + zoff;
-#pragma unroll
-    for (int f = 0; f < 5; ++f) dma_dword(mkrsrc(pr + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::G + f));
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * nn, pb), boff, lds0);
     if (dp.dens) {
       const float* uo = (const float*)a.uin + zoff;
-#pragma unroll
-      for (int f = 0; f < 3; ++f) dma_dword(mkrsrc(uo + (size_t)f * nn, pb), boff, lds0 + 256u * (SL::UO + f));
+      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * nn, pb), boff, lds0);
+      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * nn, pb), boff, lds0);
+      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * nn, pb), boff, lds0);
     }
   }
 }
@@ -327,11 +331,11 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   R lo[10], mid[10], hi[10];
 #pragma unroll
   for (int m = 0; m < 10; ++m) { lo[m] = 0; mid[m] = 0; hi[m] = 0; }
-  auto store_lo = [&](int zo) {
+  auto store_lo = [&](int zo, size_t fs) {
     Mom<R> o;
     o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
     o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
-    store_mom_b<PREC>(Sout + (size_t)zo * nxny, nn, pbT, bT, o, dp);
+    store_mom_b<PREC>(Sout + (size_t)zo * nxny, fs, pbT, bT, o, dp);
   };
 
   const R third = (R)(1.0 / 3.0), twothird = (R)(2.0 / 3.0);
@@ -340,7 +344,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   int isf_next = 1;
   if constexpr (PF) {
     lds0 = lds_addr(&sPF[wave][0][0]);
-    dma_raw<TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, pbT, bT, lds0);
+    dma_raw<TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, nn, pbT, bT, lds0);
     if (dp.obst) isf_next = isf_load(z0 - 1);
   }
 
@@ -348,6 +352,11 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
     const int zw = wrapg(zsrc, nz);
     const size_t zoff = (size_t)zw * nxny;
     const bool own_plane = zsrc >= z0 && zsrc < zend;
+    // the field stride, opaque to the optimiser inside the loop: the ~30
+    // per-field plane offsets are then rebuilt by scalar ALU each iteration
+    // instead of being hoisted and spilled to vector-register lanes
+    size_t fs = nn;
+    asm volatile("" : "+s"(fs));
 
     RawNode<PREC> cur;
     int8_t isf = 1;
@@ -357,13 +366,13 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       asm volatile("" : "+v"(t));  // consume the prefetched isfluid before this iteration's stores
       isf = (int8_t)t;
       // the plane completed by the previous iteration
-      if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2);
+      if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2, fs);
 #pragma unroll
-      for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+      for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
       lds_raw<TWO, INLINE>(cur, sPF[wave], lane, dp.dens);
     } else {
       if (dp.obst) isf = (int8_t)isf_load(zsrc);
-      load_raw<PREC, TWO, INLINE>(cur, a, zoff, pbT, pbA, bT, bA);
+      load_raw<PREC, TWO, INLINE>(cur, a, zoff, fs, pbT, pbA, bT, bA);
     }
 
     // ---- 1. decode + (inline) moments stage + collision ---------------------
@@ -417,15 +426,15 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
           if constexpr (TWO) {
             if (dp.dens) {
               T* uo = (T*)a.uout + zoff;
-              fst(uo, nn, 0, pbT, bT, C::enc(m.u, dp.scale[1]));
-              fst(uo, nn, 1, pbT, bT, C::enc(m.v, dp.scale[2]));
-              fst(uo, nn, 2, pbT, bT, C::enc(m.w, dp.scale[3]));
+              fst(uo, fs, 0, pbT, bT, C::enc(m.u, dp.scale[1]));
+              fst(uo, fs, 1, pbT, bT, C::enc(m.v, dp.scale[2]));
+              fst(uo, fs, 2, pbT, bT, C::enc(m.w, dp.scale[3]));
             }
             // update_phi (update_phi:88-93), u after the moments stage
             const R t = -ph.sharp_c * dvt;
             const R po = phl - m.u * (R)0.5 * (gfx) - m.v * (R)0.5 * (gfy) - m.w * (R)0.5 * (gfz) +
                          ph.tau_diff * lap + t;
-            fst((TA*)a.phiout + zoff, nn, 0, pbA, bA, (TA)po);
+            fst((TA*)a.phiout + zoff, fs, 0, pbA, bA, (TA)po);
           }
           if (dp.obst && !dp.bb && isf == -1) {
             // LB_int_boundary (boundary:117-212) on the post-moments state
@@ -467,7 +476,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
         // every store of this iteration is issued: stream the next plane in
         lds_wait_all();
         if (zsrc < zend) {
-          dma_raw<TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, pbT, bT, lds0);
+          dma_raw<TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, fs, pbT, bT, lds0);
           if (dp.obst) isf_next = isf_load(zsrc + 1);
         }
       }
@@ -509,8 +518,8 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
     R G0[9];  // q = 0 partial sums kept in registers: [d][r]
     if (!halo_wave) {
       // ---- 3. x: shift by one lane, contract over p ---------------------------
-      // the 18 x-halo values of this row: lane k < 18 holds sHX[wave][k / 9][k % 9]
-      const R hx = (&sHX[wave][0][0])[lane < 18 ? lane : 17];
+      // the x-halo values enter as DPP fill operands, read by every lane
+      // from one LDS address (a broadcast)
       const bool short_row = XW < 64;  // the right halo enters below lane 63
       R G[3][3][3];  // [d][q][r]
 #pragma unroll
@@ -519,11 +528,9 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
         for (int q = -1; q <= 1; ++q) {
           const int qr = (q + 1) + 3 * (r + 1);
           R fm_in = 0, fp_in = 0;
-          if (present<LAT>(1, q, r)) {
-            fp_in = shift_from_lower(f[cidx(1, q, r)], lane_value(hx, qr));
-          }
+          if (present<LAT>(1, q, r)) fp_in = shift_from_lower(f[cidx(1, q, r)], sHX[wave][0][qr]);
           if (present<LAT>(-1, q, r)) {
-            const R hr = lane_value(hx, 9 + qr);
+            const R hr = sHX[wave][1][qr];
             fm_in = shift_from_upper(f[cidx(-1, q, r)], hr);
             if (short_row) fm_in = lane == XW - 1 ? hr : fm_in;
           }
@@ -570,16 +577,18 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       // ---- 5. z: r = +1 -> plane zsrc+1 (hi), 0 -> zsrc (mid), -1 -> zsrc-1 (lo)
       // outputs: 0 P(000) 1 u(100) 2 v(010) 3 w(001) 4 xx(200) 5 yy(020) 6 zz(002)
       //          7 xy(110) 8 xz(101) 9 yz(011); index [0]=r-1 [1]=r0 [2]=r+1
-      hi[0] += H00[2]; mid[0] += H00[1]; lo[0] += H00[0];
-      hi[3] += H00[2]; lo[3] -= H00[0];
-      hi[6] += twothird * H00[2]; mid[6] -= third * H00[1]; lo[6] += twothird * H00[0];
-      hi[2] += H01[2]; mid[2] += H01[1]; lo[2] += H01[0];
-      hi[9] += H01[2]; lo[9] -= H01[0];
-      hi[1] += H10[2]; mid[1] += H10[1]; lo[1] += H10[0];
-      hi[8] += H10[2]; lo[8] -= H10[0];
-      hi[5] += H02[2]; mid[5] += H02[1]; lo[5] += H02[0];
-      hi[4] += H20[2]; mid[4] += H20[1]; lo[4] += H20[0];
-      hi[7] += H11[2]; mid[7] += H11[1]; lo[7] += H11[0];
+      // plane zsrc+1 receives its first term here: assign rather than add to
+      // a zero (0 + x cannot be folded for signed zeros)
+      hi[0] = H00[2]; mid[0] += H00[1]; lo[0] += H00[0];
+      hi[3] = H00[2]; lo[3] -= H00[0];
+      hi[6] = twothird * H00[2]; mid[6] -= third * H00[1]; lo[6] += twothird * H00[0];
+      hi[2] = H01[2]; mid[2] += H01[1]; lo[2] += H01[0];
+      hi[9] = H01[2]; lo[9] -= H01[0];
+      hi[1] = H10[2]; mid[1] += H10[1]; lo[1] += H10[0];
+      hi[8] = H10[2]; lo[8] -= H10[0];
+      hi[5] = H02[2]; mid[5] += H02[1]; lo[5] += H02[0];
+      hi[4] = H20[2]; mid[4] += H20[1]; lo[4] += H20[0];
+      hi[7] = H11[2]; mid[7] += H11[1]; lo[7] += H11[0];
       if (has_lbc) {
 #pragma unroll
         for (int q = 0; q < 10; ++q) mid[q] += lbc[q];
@@ -587,14 +596,14 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
       // ---- 6. plane zsrc-1 is complete (stored here, or by the next iteration) --
       if constexpr (!PF) {
         const int zo = zsrc - 1;
-        if (lane < XW && zo >= z0 && zo < zend) store_lo(zo);
+        if (lane < XW && zo >= z0 && zo < zend) store_lo(zo, fs);
 #pragma unroll
-        for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; hi[m] = 0; }
+        for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
       }
     }
   }
   if constexpr (PF) {
-    if (out_row && lane < XW) store_lo(zend - 1);
+    if (out_row && lane < XW) store_lo(zend - 1, nn);
   }
 }
 
