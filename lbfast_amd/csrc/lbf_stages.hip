@@ -319,6 +319,11 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
     return v == 1 || v == -1;
   };
 
+  // outputs held for one iteration (see below)
+  R hg[2][4], hd[2];
+  size_t og[2] = {0, 0}, od[2] = {0, 0};
+  bool st_g[2] = {false, false}, st_d[2] = {false, false};
+
   dma_plane(z0 - 2);
   dma_plane(z0 - 1);
   dma_plane(z0);
@@ -331,12 +336,31 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
       dma_wait_all();
     __syncthreads();
 
-    // ---- grad / lap / theta of plane k on theta rows t (y = y0 - 1 + t) ------
+    // ---- the previous iteration's outputs ---------------------------------------
+    // Stores are issued here, a full plane before the next wait on the
+    // load/store counter, instead of just ahead of it.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (st_g[h]) {
+        pre[og[h]] = (TA)hg[h][0];
+        pre[og[h] + nn] = (TA)hg[h][1];
+        pre[og[h] + 2 * nn] = (TA)hg[h][2];
+        pre[og[h] + 3 * nn] = (TA)hg[h][3];
+      }
+      if (st_d[h]) pre[od[h] + 4 * nn] = (TA)hd[h];
+      st_g[h] = false;
+      st_d[h] = false;
+    }
+
+    // ---- grad / lap / theta of plane k on theta rows wave, wave + QW -------------
+    static_assert(QT == 2 * QW, "two theta rows per wave");
     {
       const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
       const int tk = (k + 3) % 3;
       const bool own = k >= z0 && k < zend;
-      for (int t = wave; t < QT; t += QW) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int t = wave + QW * h;
         // lanes 0 and 63 read their own column for the missing neighbour;
         // no output depends on them
 #define F(a_, b_, c_) ((R)sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][t + 1 + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)])
@@ -350,14 +374,14 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
           const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
           thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
           if (own && out_lane && t >= 1 && t <= QY && y < ny) {
-            const R lap = LAPL(F, c);
             // the reference re-reads n and |g| from storage precision
             const R mgs = (R)(TA)mg;
-            const size_t o = (size_t)k * nxny + (size_t)y * nx + xl;
-            pre[o] = (TA)((R)(TA)nxv * mgs);
-            pre[o + nn] = (TA)((R)(TA)nyv * mgs);
-            pre[o + 2 * nn] = (TA)((R)(TA)nzv * mgs);
-            pre[o + 3 * nn] = (TA)lap;
+            hg[h][0] = (R)(TA)nxv * mgs;
+            hg[h][1] = (R)(TA)nyv * mgs;
+            hg[h][2] = (R)(TA)nzv * mgs;
+            hg[h][3] = LAPL(F, c);
+            og[h] = (size_t)k * nxny + (size_t)y * nx + xl;
+            st_g[h] = true;
           }
         }
 #undef F
@@ -368,11 +392,14 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
     }
     __syncthreads();
 
-    // ---- div theta of plane k-1 on theta rows 1..QY -----------------------------
+    // ---- div theta of plane k-1 on theta rows 1 + wave, 1 + wave + QW ------------
     const int zo = k - 1;
     if (zo >= z0 && zo < zend) {
       const int tm = (k + 1) % 3, t0 = (k + 2) % 3, tp = (k + 3) % 3;  // planes k-2, k-1, k
-      for (int t = 1 + wave; t <= QY; t += QW) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int t = 1 + wave + QW * h;
+        if (t > QY) continue;
 #define TH(f_, a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][f_][t + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)]
 #define X(a_, b_, c_) TH(0, a_, b_, c_)
 #define Y(a_, b_, c_) TH(1, a_, b_, c_)
@@ -383,11 +410,25 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
 #undef Z
 #undef TH
         const int y = y0 - 1 + t;
-        if (out_lane && y < ny && fluid(xl, y, zo)) pre[(size_t)zo * nxny + (size_t)y * nx + xl + 4 * nn] = (TA)d;
+        if (out_lane && y < ny && fluid(xl, y, zo)) {
+          hd[h] = d;
+          od[h] = (size_t)zo * nxny + (size_t)y * nx + xl;
+          st_d[h] = true;
+        }
       }
     }
-    // every store of this iteration is issued; slot (k+3)&3 held plane k-1, read above
+    // slot (k+3)&3 held plane k-1, read above
     if (k + 3 <= zend + 1) dma_plane(k + 3);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (st_g[h]) {
+      pre[og[h]] = (TA)hg[h][0];
+      pre[og[h] + nn] = (TA)hg[h][1];
+      pre[og[h] + 2 * nn] = (TA)hg[h][2];
+      pre[og[h] + 3 * nn] = (TA)hg[h][3];
+    }
+    if (st_d[h]) pre[od[h] + 4 * nn] = (TA)hd[h];
   }
 }
 
