@@ -33,6 +33,8 @@ def test_library_exports_every_declared_symbol():
 def test_params_struct_layout_matches_header():
     # offsets of include/lbfast_hip.h's lbf_params as laid out by the C compiler
     assert (LbfParams.fx.offset, LbfParams.tau1.offset, LbfParams.store_scale.offset) == (72, 160, 208)
+    assert (LbfParams.bounce_back.offset, LbfParams.smagorinsky.offset, LbfParams.repulsive_flux.offset) == (56, 60, 64)
+    assert (LbfParams.a_rep.offset, LbfParams.q_th.offset, LbfParams.cos_opp_t.offset) == (288, 296, 304)
     assert ctypes.sizeof(LbfParams) == 320
     L = hip.load()
     p = LbfParams()
