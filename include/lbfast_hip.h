@@ -208,6 +208,37 @@ int lbf_set_tile(lbf_ctx* ctx, int32_t tile_y, int32_t tile_z);
 
 int32_t lbf_abi_version(void);
 
+/* ---- batches of independent domains ----------------------------------------
+ * No counterpart in the reference, which runs one domain per process.  Many
+ * independent domains (initialised contexts sharing lattice, HIGHORDER,
+ * precision and component count; any sizes and physical parameters; no
+ * obstacles or REPULSIVE_FLUX) advance together: each step is one phase-field
+ * launch and one fused launch whose grids cover every domain's tiles, so that
+ * small domains fill the GPU.  The batch borrows the contexts (they must
+ * outlive it) and keeps their step counters and buffers current, so uploads,
+ * downloads and single-domain calls stay valid between batch calls; work on
+ * a context's stream is ordered before and after each batch step.  Two-
+ * component batches need float phase-field storage (not FP64). */
+typedef struct lbf_batch lbf_batch;
+int lbf_batch_create(lbf_ctx* const* ctxs, int32_t ndomains, lbf_batch** out);
+void lbf_batch_destroy(lbf_batch* b);
+const char* lbf_batch_last_error(const lbf_batch* b);
+/* Advance every domain nsteps (stream-ordered; replays a captured graph for
+ * whole multiples of its length, launches the rest directly). */
+int lbf_batch_step(lbf_batch* b, int32_t nsteps);
+/* Capture nsteps (even, >= 2) steps into a hipGraph that later
+ * lbf_batch_step calls replay; set_tile and autotune discard it. */
+int lbf_batch_capture(lbf_batch* b, int32_t nsteps);
+/* Row tile (8 or 13) of the fused kernel and z chunks of the fused and
+ * phase-field kernels, for every domain (chunks clipped to each nz). */
+int lbf_batch_set_tile(lbf_batch* b, int32_t tile_y, int32_t tile_z, int32_t tile_z_prep);
+/* Time trial_steps (even) steps of each candidate tile on the batch itself,
+ * keep the fastest and restore every domain's state; reports the choice. */
+int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* tile_y, int32_t* tile_z);
+int lbf_batch_synchronize(lbf_batch* b);
+/* Elapsed milliseconds of the last lbf_batch_step call (HIP events). */
+int lbf_batch_timing(lbf_batch* b, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

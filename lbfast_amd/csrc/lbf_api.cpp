@@ -716,4 +716,329 @@ int lbf_synchronize(lbf_ctx* c) {
   return LBF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batches: many independent domains advanced by one prep launch and one fused
+// launch per step (or a captured graph of such steps).  The contexts keep
+// their state and stay usable for uploads/downloads between batch calls.
+struct lbf_batch {
+  std::vector<lbf_ctx*> ctx;
+  int prec = 0, lattice = 27;
+  bool ho = false, two = false;
+  int ty = 13;
+  std::vector<int> tz, tz_prep;
+  FusedBatchEntry* ftab[2] = {};
+  PrepBatchEntry* ptab[2] = {};
+  int ftotal[2] = {}, ptotal[2] = {};
+  int par = 0;  // 0: tables built from the contexts' current buffers
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  std::vector<hipEvent_t> cev;  // one per context: orders its stream before batch work
+  bool timing_valid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  std::string err;
+};
+
+static int bfail(lbf_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf; else g_err = buf;
+  return code;
+}
+#define BHIPCHK(b, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return bfail((b), LBF_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+static void batch_drop_graph(lbf_batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+// Device tables for both buffer parities (parity 1 = after one step).
+static int batch_tables(lbf_batch* b) {
+  const int nd = (int)b->ctx.size();
+  for (int par = 0; par < 2; ++par) {
+    std::vector<FusedBatchEntry> f(nd);
+    std::vector<PrepBatchEntry> q(nd);
+    int ft = 0, pt = 0;
+    for (int d = 0; d < nd; ++d) {
+      lbf_ctx* c = b->ctx[d];
+      const int cur0 = c->cur, ucur0 = c->ucur;
+      c->cur ^= par;
+      c->ucur ^= par;
+      f[d].a = fused_args(c);
+      f[d].a.ty = b->ty;
+      f[d].a.tz = b->tz[d];
+      f[d].tile0 = ft;
+      ft += fused_tile_count(f[d].a);
+      q[d].a = stage_args(c);
+      q[d].a.gf_from_pre = 1;
+      q[d].tz = b->tz_prep[d];
+      q[d].tile0 = pt;
+      if (b->two) pt += prep_tile_count(q[d].a, b->prec, b->tz_prep[d]);
+      c->cur = cur0;
+      c->ucur = ucur0;
+    }
+    if (!b->ftab[par]) BHIPCHK(b, hipMalloc(&b->ftab[par], nd * sizeof(FusedBatchEntry)));
+    if (!b->ptab[par]) BHIPCHK(b, hipMalloc(&b->ptab[par], nd * sizeof(PrepBatchEntry)));
+    BHIPCHK(b, hipMemcpy(b->ftab[par], f.data(), nd * sizeof(FusedBatchEntry), hipMemcpyHostToDevice));
+    BHIPCHK(b, hipMemcpy(b->ptab[par], q.data(), nd * sizeof(PrepBatchEntry), hipMemcpyHostToDevice));
+    b->ftotal[par] = ft;
+    b->ptotal[par] = pt;
+  }
+  return LBF_OK;
+}
+
+// Enqueue one step on b->stream at the batch's current parity (no state change).
+static int batch_enqueue(lbf_batch* b, int par) {
+  const int nd = (int)b->ctx.size();
+  if (b->two) BHIPCHK(b, launch_phase_prep_batch(b->ptab[par], nd, b->ptotal[par], b->prec, b->stream));
+  BHIPCHK(b, launch_fused_batch(b->ftab[par], nd, b->ftotal[par], b->prec, b->lattice, b->ho, b->two, b->ty, b->stream));
+  return LBF_OK;
+}
+
+static void batch_advance(lbf_batch* b, int nsteps) {
+  for (lbf_ctx* c : b->ctx) {
+    if (nsteps & 1) {
+      c->cur ^= 1;
+      c->ucur ^= 1;
+    }
+    c->step += nsteps;
+  }
+}
+
+int lbf_batch_create(lbf_ctx* const* ctxs, int32_t ndomains, lbf_batch** out) {
+  if (!ctxs || !out || ndomains < 1) return bfail(nullptr, LBF_EINVAL, "bad arguments");
+  *out = nullptr;
+  lbf_ctx* c0 = ctxs[0];
+  if (!c0) return bfail(nullptr, LBF_EINVAL, "null context");
+  for (int d = 0; d < ndomains; ++d) {
+    lbf_ctx* c = ctxs[d];
+    if (!c || !c->initialized) return bfail(nullptr, LBF_ESTATE, "domain %d: context missing or not initialised", d);
+    if (c->prec != c0->prec || c->p.lattice != c0->p.lattice || c->dp.highorder != c0->dp.highorder ||
+        c->p.twocomponent != c0->p.twocomponent || c->device != c0->device)
+      return bfail(nullptr, LBF_EINVAL, "domain %d: a batch shares lattice, HIGHORDER, precision, component count and device", d);
+    if (c->p.internal_obstacles || c->p.repulsive_flux)
+      return bfail(nullptr, LBF_EINVAL, "domain %d: obstacles and REPULSIVE_FLUX run per domain, not batched", d);
+    if (c->p.twocomponent && c->prec == LBF_PREC_FP64)
+      return bfail(nullptr, LBF_EINVAL, "two-component batches need float phase-field storage (not FP64)");
+    for (int e = 0; e < d; ++e)
+      if (ctxs[e] == c) return bfail(nullptr, LBF_EINVAL, "domain %d repeats domain %d", d, e);
+  }
+  lbf_batch* b = new lbf_batch();
+  b->ctx.assign(ctxs, ctxs + ndomains);
+  b->prec = c0->prec;
+  b->lattice = c0->p.lattice;
+  b->ho = c0->dp.highorder != 0;
+  b->two = c0->p.twocomponent != 0;
+  b->ty = 13;
+  for (lbf_ctx* c : b->ctx) {
+    b->tz.push_back(c->p.nz < 32 ? c->p.nz : 32);
+    b->tz_prep.push_back(c->p.nz < 32 ? c->p.nz : 32);
+  }
+  bool ok = hipStreamCreate(&b->stream) == hipSuccess && hipEventCreate(&b->t0) == hipSuccess &&
+            hipEventCreate(&b->t1) == hipSuccess;
+  for (lbf_ctx* c : b->ctx) {
+    hipEvent_t e = nullptr;
+    ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    b->cev.push_back(e);
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+  }
+  if (!ok) {
+    lbf_batch_destroy(b);
+    return bfail(nullptr, LBF_EHIP, "stream/event creation failed");
+  }
+  int rc = batch_tables(b);
+  if (rc) {
+    g_err = b->err;
+    lbf_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return LBF_OK;
+}
+
+void lbf_batch_destroy(lbf_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (int i = 0; i < 2; ++i) {
+    if (b->ftab[i]) hipFree(b->ftab[i]);
+    if (b->ptab[i]) hipFree(b->ptab[i]);
+  }
+  if (b->t0) hipEventDestroy(b->t0);
+  if (b->t1) hipEventDestroy(b->t1);
+  for (hipEvent_t e : b->cev)
+    if (e) hipEventDestroy(e);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* lbf_batch_last_error(const lbf_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+int lbf_batch_set_tile(lbf_batch* b, int32_t ty, int32_t tz, int32_t tz_prep) {
+  if (!b) return LBF_EINVAL;
+  if (ty != 8 && ty != 13) return bfail(b, LBF_EINVAL, "tile_y must be 8 or 13");
+  if (tz < 1 || tz_prep < 1) return bfail(b, LBF_EINVAL, "z chunks must be >= 1");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  b->ty = ty;
+  for (size_t d = 0; d < b->ctx.size(); ++d) {
+    const int nz = b->ctx[d]->p.nz;
+    b->tz[d] = tz < nz ? tz : nz;
+    b->tz_prep[d] = tz_prep < nz ? tz_prep : nz;
+  }
+  batch_drop_graph(b);
+  // tables describe the buffers at the batch's parity 0: rebuild from the contexts' state
+  b->par = 0;
+  return batch_tables(b);
+}
+
+int lbf_batch_step(lbf_batch* b, int32_t nsteps) {
+  if (!b) return LBF_EINVAL;
+  if (nsteps < 0) return bfail(b, LBF_EINVAL, "nsteps < 0");
+  // work queued on a domain's own stream (uploads) comes first ...
+  for (size_t d = 0; d < b->ctx.size(); ++d) {
+    BHIPCHK(b, hipEventRecord(b->cev[d], b->ctx[d]->stream));
+    BHIPCHK(b, hipStreamWaitEvent(b->stream, b->cev[d], 0));
+  }
+  BHIPCHK(b, hipEventRecord(b->t0, b->stream));
+  int done = 0;
+  if (b->gexec && b->graph_steps > 0) {
+    while (nsteps - done >= b->graph_steps) {
+      BHIPCHK(b, hipGraphLaunch(b->gexec, b->stream));
+      done += b->graph_steps;  // even: parity unchanged
+    }
+  }
+  for (; done < nsteps; ++done) {
+    int rc = batch_enqueue(b, b->par);
+    if (rc) return rc;
+    b->par ^= 1;
+  }
+  BHIPCHK(b, hipEventRecord(b->t1, b->stream));
+  // ... and work later queued on it (downloads, single-domain steps) comes after
+  for (lbf_ctx* c : b->ctx) BHIPCHK(b, hipStreamWaitEvent(c->stream, b->t1, 0));
+  batch_advance(b, nsteps);
+  b->timing_valid = nsteps > 0;
+  // keep tables indexed from the contexts' current buffers
+  if (b->par) {
+    FusedBatchEntry* f = b->ftab[0]; b->ftab[0] = b->ftab[1]; b->ftab[1] = f;
+    PrepBatchEntry* q = b->ptab[0]; b->ptab[0] = b->ptab[1]; b->ptab[1] = q;
+    int t = b->ftotal[0]; b->ftotal[0] = b->ftotal[1]; b->ftotal[1] = t;
+    t = b->ptotal[0]; b->ptotal[0] = b->ptotal[1]; b->ptotal[1] = t;
+    b->par = 0;
+    batch_drop_graph(b);  // a captured graph refers to the old parity's tables
+  }
+  return LBF_OK;
+}
+
+int lbf_batch_capture(lbf_batch* b, int32_t nsteps) {
+  if (!b) return LBF_EINVAL;
+  if (nsteps < 2 || (nsteps & 1)) return bfail(b, LBF_EINVAL, "a captured graph holds an even number of steps (>= 2)");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < nsteps && !rc; ++s) rc = batch_enqueue(b, s & 1);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return bfail(b, LBF_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = g;
+  e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    batch_drop_graph(b);
+    return bfail(b, LBF_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  b->graph_steps = nsteps;
+  return LBF_OK;
+}
+
+int lbf_batch_synchronize(lbf_batch* b) {
+  if (!b) return LBF_EINVAL;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return LBF_OK;
+}
+
+int lbf_batch_timing(lbf_batch* b, double* ms) {
+  if (!b || !ms) return LBF_EINVAL;
+  if (!b->timing_valid) return bfail(b, LBF_ESTATE, "no timed lbf_batch_step call");
+  BHIPCHK(b, hipEventSynchronize(b->t1));
+  float m = 0;
+  BHIPCHK(b, hipEventElapsedTime(&m, b->t0, b->t1));
+  *ms = m;
+  return LBF_OK;
+}
+
+// The project's autotuner: time `trial_steps` (even) steps of each candidate
+// tile (row tile 8 or 13; z chunks 16..128 for both kernels) on the batch
+// itself, keep the fastest, and restore every domain's state afterwards.
+int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int32_t* best_tz) {
+  if (!b) return LBF_EINVAL;
+  if (trial_steps < 2 || (trial_steps & 1)) return bfail(b, LBF_EINVAL, "trial_steps must be even and >= 2");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (lbf_ctx* c : b->ctx) BHIPCHK(b, hipStreamSynchronize(c->stream));
+  // snapshot: the S moments, u_old and phi each step reads
+  std::vector<void*> snap;
+  auto release = [&]() { for (void* q : snap) hipFree(q); snap.clear(); };
+  for (lbf_ctx* c : b->ctx) {
+    const size_t bytes[3] = {10 * c->N * c->tsz, 3 * c->N * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
+    void* src[3] = {c->h[c->cur], c->U[c->ucur], c->p.twocomponent ? c->phi[c->cur] : nullptr};
+    for (int i = 0; i < 3; ++i) {
+      void* q = nullptr;
+      if (bytes[i]) {
+        hipError_t e = hipMalloc(&q, bytes[i]);
+        if (e != hipSuccess) { release(); return bfail(b, LBF_EHIP, "autotune snapshot: %s", hipGetErrorString(e)); }
+        e = hipMemcpyAsync(q, src[i], bytes[i], hipMemcpyDeviceToDevice, b->stream);
+        if (e != hipSuccess) { hipFree(q); release(); return bfail(b, LBF_EHIP, "autotune snapshot copy"); }
+      }
+      snap.push_back(q);
+    }
+  }
+  const int tys[2] = {8, 13}, tzs[4] = {16, 32, 64, 128};
+  double best = 1e300;
+  int bty = b->ty, btz = b->tz.empty() ? 32 : b->tz[0];
+  for (int ty : tys)
+    for (int tz : tzs) {
+      int rc = lbf_batch_set_tile(b, ty, tz, tz);
+      if (rc) { release(); return rc; }
+      for (int s = 0; s < 2 && !rc; ++s) rc = batch_enqueue(b, s & 1);  // warm-up pair
+      BHIPCHK(b, hipEventRecord(b->t0, b->stream));
+      for (int s = 0; s < trial_steps && !rc; ++s) rc = batch_enqueue(b, s & 1);
+      BHIPCHK(b, hipEventRecord(b->t1, b->stream));
+      if (rc) { release(); return rc; }
+      BHIPCHK(b, hipEventSynchronize(b->t1));
+      float m = 0;
+      BHIPCHK(b, hipEventElapsedTime(&m, b->t0, b->t1));
+      if (m < best) { best = m; bty = ty; btz = tz; }
+    }
+  // restore the snapshot (trial steps were even: the parities are unchanged)
+  size_t k = 0;
+  for (lbf_ctx* c : b->ctx) {
+    const size_t bytes[3] = {10 * c->N * c->tsz, 3 * c->N * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
+    void* dst[3] = {c->h[c->cur], c->U[c->ucur], c->p.twocomponent ? c->phi[c->cur] : nullptr};
+    for (int i = 0; i < 3; ++i, ++k)
+      if (bytes[i]) BHIPCHK(b, hipMemcpyAsync(dst[i], snap[k], bytes[i], hipMemcpyDeviceToDevice, b->stream));
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  release();
+  int rc = lbf_batch_set_tile(b, bty, btz, btz);
+  if (rc) return rc;
+  if (best_ty) *best_ty = bty;
+  if (best_tz) *best_tz = btz;
+  b->timing_valid = false;
+  return LBF_OK;
+}
+
 }  // extern "C"

@@ -265,8 +265,14 @@ __device__ __forceinline__ void lds_raw(RawNode<PREC_FP32>& r, const float (*slo
 // shared load/store counter makes a wait for everything) finds the older
 // stores long done.  The completed output plane is therefore stored at the
 // top of the following iteration.
+template <int TY>
+__host__ __device__ inline int fused_tiles(const FusedArgs& a) {
+  return ((a.dp.nx + 63) / 64) * ((a.dp.ny + TY - 1) / TY) * ((a.dp.nz + a.tz - 1) / a.tz);
+}
+
+// One block: tile `tile` (x fastest, then y, then z chunks) of the domain `a`.
 template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
-__global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
+__device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef Codec<PREC> C;
   typedef typename C::R R;
   typedef typename C::T T;
@@ -285,9 +291,7 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nn = dp.nn, nxny = (size_t)nx * ny;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  // 1-D grid, XCD-aware tile order (tx fastest)
-  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY, ntz = (nz + a.tz - 1) / a.tz;
-  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tzi = tile / (ntx * nty);
   const int x0 = tx * 64;
   const int XW = min(64, nx - x0);
@@ -607,10 +611,31 @@ __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
   }
 }
 
+// One domain per launch: 1-D grid, XCD-aware tile order.
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
+__global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
+  fused_body<LAT, HO, PREC, TWO, INLINE, TY>(a, xcd_tile(blockIdx.x, fused_tiles<TY>(a)));
+}
+
+// Many domains per launch: the grid covers every domain's tiles in table
+// order (domain-major); a block finds its domain by binary search over the
+// tile offsets, then runs the same body on that domain's arguments.
+template <int LAT, bool HO, int PREC, bool TWO, int TY>
+__global__ void __launch_bounds__((TY + 3) * 64)
+    fused_kernel_batch(const FusedBatchEntry* __restrict__ tab, int ndom, int total) {
+  const int g = xcd_tile(blockIdx.x, total);
+  int lo = 0, hi = ndom - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].tile0 <= g) lo = mid; else hi = mid - 1;
+  }
+  fused_body<LAT, HO, PREC, TWO, true, TY>(tab[lo].a, g - tab[lo].tile0);
+}
+
 // ---------------------------------------------------------------------------
 template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
 static hipError_t launch_t(const FusedArgs& a, hipStream_t s) {
-  dim3 grid(((a.dp.nx + 63) / 64) * ((a.dp.ny + TY - 1) / TY) * ((a.dp.nz + a.tz - 1) / a.tz));
+  dim3 grid(fused_tiles<TY>(a));
   dim3 block((TY + 3) * 64);
   hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY>), grid, block, 0, s, a);
   return hipGetLastError();
@@ -636,6 +661,43 @@ static hipError_t launch_ty(const FusedArgs& a, hipStream_t s) {
     case 13: return launch_two<PREC, INLINE, 13>(a, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+int fused_tile_count(const FusedArgs& a) {
+  switch (a.ty) {
+    case 8: return fused_tiles<8>(a);
+    case 13: return fused_tiles<13>(a);
+  }
+  return -1;
+}
+
+template <int PREC, bool TWO, int TY>
+static hipError_t batch_lat(const FusedBatchEntry* tab, int ndom, int total, int lat, bool ho, hipStream_t s) {
+  dim3 grid(total), block((TY + 3) * 64);
+  if (lat == 19) hipLaunchKernelGGL((fused_kernel_batch<19, false, PREC, TWO, TY>), grid, block, 0, s, tab, ndom, total);
+  else if (ho) hipLaunchKernelGGL((fused_kernel_batch<27, true, PREC, TWO, TY>), grid, block, 0, s, tab, ndom, total);
+  else hipLaunchKernelGGL((fused_kernel_batch<27, false, PREC, TWO, TY>), grid, block, 0, s, tab, ndom, total);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t batch_prec(const FusedBatchEntry* tab, int ndom, int total, int lat, bool ho, bool two, int ty,
+                             hipStream_t s) {
+  if (ty == 8) return two ? batch_lat<PREC, true, 8>(tab, ndom, total, lat, ho, s) : batch_lat<PREC, false, 8>(tab, ndom, total, lat, ho, s);
+  if (ty == 13) return two ? batch_lat<PREC, true, 13>(tab, ndom, total, lat, ho, s) : batch_lat<PREC, false, 13>(tab, ndom, total, lat, ho, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused_batch(const FusedBatchEntry* dev_tab, int ndom, int total_tiles, int prec, int lattice,
+                              bool highorder, bool two, int ty, hipStream_t s) {
+  switch (prec) {
+    case PREC_FP64: return batch_prec<PREC_FP64>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+    case PREC_FP32: return batch_prec<PREC_FP32>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+    case PREC_MIXED: return batch_prec<PREC_MIXED>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+    case PREC_FP16S: return batch_prec<PREC_FP16S>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+    case PREC_BF16S: return batch_prec<PREC_BF16S>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s) {

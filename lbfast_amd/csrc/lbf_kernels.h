@@ -38,7 +38,24 @@ struct StageArgs {
   DevParams dp;
 };
 
+// Batched launches: one grid over many domains (same lattice, precision,
+// component count and row tile; any sizes and physical parameters).
+struct FusedBatchEntry {
+  FusedArgs a;
+  int tile0;  // first global tile of this domain
+};
+struct PrepBatchEntry {
+  StageArgs a;
+  int tz;     // z-chunk of the phase-field prep
+  int tile0;
+};
+
 hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s);
+int fused_tile_count(const FusedArgs& a);
+hipError_t launch_fused_batch(const FusedBatchEntry* dev_tab, int ndom, int total_tiles, int prec, int lattice,
+                              bool highorder, bool two, int ty, hipStream_t s);
+int prep_tile_count(const StageArgs& a, int prec, int tz);
+hipError_t launch_phase_prep_batch(const PrepBatchEntry* dev_tab, int ndom, int total_tiles, int prec, hipStream_t s);
 hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s);
 
 // storage conversion: host-order double/float array -> field (codec-aware)

@@ -269,8 +269,12 @@ constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
 constexpr int QT = QY + 2;                 // theta rows (1-row halo)
 constexpr int QDMA = (QR + QW - 1) / QW;   // phi rows each wave streams per plane
 
+__host__ __device__ inline int prep_tiles_z(const DevParams& dp, int tz) {
+  return ((dp.nx + QX - 1) / QX) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
+}
+
 template <int PREC>
-__global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
+__device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile) {
   typedef typename Codec<PREC>::R R;
   typedef typename Aux<PREC>::T TA;
   static_assert(sizeof(TA) == 4, "float phase-field storage");
@@ -281,8 +285,7 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nn = dp.nn, nxny = (size_t)nx * ny;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int ntx = (nx + QX - 1) / QX, nty = (ny + QY - 1) / QY, ntz = (nz + tz - 1) / tz;
-  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int ntx = (nx + QX - 1) / QX, nty = (ny + QY - 1) / QY;
   const int x0 = (tile % ntx) * QX, y0 = ((tile / ntx) % nty) * QY;
   const int z0 = (tile / (ntx * nty)) * tz, zend = min(z0 + tz, nz);
   const int xl = x0 - 2 + lane;
@@ -430,6 +433,24 @@ __global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
     }
     if (st_d[h]) pre[od[h] + 4 * nn] = (TA)hd[h];
   }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
+  prep_body_z<PREC>(a, tz, xcd_tile(blockIdx.x, prep_tiles_z(a.dp, tz)));
+}
+
+// many domains per launch (see fused_kernel_batch)
+template <int PREC>
+__global__ void __launch_bounds__(QW * 64)
+    k_phase_prep_z_batch(const PrepBatchEntry* __restrict__ tab, int ndom, int total) {
+  const int g = xcd_tile(blockIdx.x, total);
+  int lo = 0, hi = ndom - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].tile0 <= g) lo = mid; else hi = mid - 1;
+  }
+  prep_body_z<PREC>(tab[lo].a, tab[lo].tz, g - tab[lo].tile0);
 }
 
 // ---------------------------------------------------------------------------
@@ -814,11 +835,27 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
 template <int PREC>
 static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
   if constexpr (sizeof(typename Aux<PREC>::T) == 4) {
-    const dim3 grid(((a.dp.nx + QX - 1) / QX) * ((a.dp.ny + QY - 1) / QY) * ((a.dp.nz + tz - 1) / tz));
-    hipLaunchKernelGGL(k_phase_prep_z<PREC>, grid, dim3(QW * 64), 0, s, a, tz);
+    hipLaunchKernelGGL(k_phase_prep_z<PREC>, dim3(prep_tiles_z(a.dp, tz)), dim3(QW * 64), 0, s, a, tz);
   } else {
     const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
     hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
+  }
+  return hipGetLastError();
+}
+
+int prep_tile_count(const StageArgs& a, int prec, int tz) {
+  if (prec == PREC_FP64) return -1;  // the FP64 prep is not batched
+  return prep_tiles_z(a.dp, tz);
+}
+
+hipError_t launch_phase_prep_batch(const PrepBatchEntry* dev_tab, int ndom, int total_tiles, int prec, hipStream_t s) {
+  const dim3 grid(total_tiles), block(QW * 64);
+  switch (prec) {
+    case PREC_FP32: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_FP32>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_MIXED: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_MIXED>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_FP16S: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_FP16S>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_BF16S: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_BF16S>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -26,6 +26,8 @@ EXPORTS = (
     "lbf_step", "lbf_begin_step", "lbf_stage", "lbf_end_step", "lbf_reduce_fp64",
     "lbf_synchronize", "lbf_get_step", "lbf_field_device_ptr", "lbf_last_step_timing",
     "lbf_set_tile",
+    "lbf_batch_create", "lbf_batch_destroy", "lbf_batch_last_error", "lbf_batch_step", "lbf_batch_capture",
+    "lbf_batch_set_tile", "lbf_batch_autotune", "lbf_batch_synchronize", "lbf_batch_timing",
 )
 
 
@@ -67,6 +69,17 @@ def load(path: str = LIB_PATH):
     L.lbf_field_device_ptr.restype = vp
     L.lbf_last_step_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.lbf_set_tile.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    i32, pi32 = ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)
+    L.lbf_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]
+    L.lbf_batch_destroy.argtypes = [vp]
+    L.lbf_batch_last_error.argtypes = [vp]
+    L.lbf_batch_last_error.restype = ctypes.c_char_p
+    L.lbf_batch_step.argtypes = [vp, i32]
+    L.lbf_batch_capture.argtypes = [vp, i32]
+    L.lbf_batch_set_tile.argtypes = [vp, i32, i32, i32]
+    L.lbf_batch_autotune.argtypes = [vp, i32, pi32, pi32]
+    L.lbf_batch_synchronize.argtypes = [vp]
+    L.lbf_batch_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -196,3 +209,56 @@ class Domain:
 
     def device_ptr(self, fid: int) -> int:
         return self.L.lbf_field_device_ptr(self.h, fid) or 0
+
+
+class Batch:
+    """Independent domains advanced together by one launch per kernel and step
+    (lbf_batch_*).  The domains must outlive the batch."""
+
+    def __init__(self, domains):
+        self.L = load()
+        self.domains = list(domains)
+        arr = (ctypes.c_void_p * len(self.domains))(*[d.h.value for d in self.domains])
+        h = ctypes.c_void_p()
+        rc = self.L.lbf_batch_create(arr, len(self.domains), ctypes.byref(h))
+        if rc:
+            raise LbfError(f"lbf_batch_create -> {rc}: {self.L.lbf_global_last_error().decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc:
+            raise LbfError(f"{what} -> {rc}: {self.L.lbf_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        self._chk(self.L.lbf_batch_step(self.h, n), "lbf_batch_step")
+
+    def capture(self, n: int):
+        self._chk(self.L.lbf_batch_capture(self.h, n), "lbf_batch_capture")
+
+    def set_tile(self, ty: int, tz: int, tz_prep: int = None):
+        self._chk(self.L.lbf_batch_set_tile(self.h, ty, tz, tz if tz_prep is None else tz_prep), "lbf_batch_set_tile")
+
+    def autotune(self, trial_steps: int = 4):
+        ty, tz = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.L.lbf_batch_autotune(self.h, trial_steps, ctypes.byref(ty), ctypes.byref(tz)),
+                  "lbf_batch_autotune")
+        return ty.value, tz.value
+
+    def synchronize(self):
+        self._chk(self.L.lbf_batch_synchronize(self.h), "lbf_batch_synchronize")
+
+    def timing(self) -> float:
+        ms = ctypes.c_double()
+        self._chk(self.L.lbf_batch_timing(self.h, ctypes.byref(ms)), "lbf_batch_timing")
+        return ms.value
