@@ -122,6 +122,57 @@ def cpu_baseline(target_s=15.0):
                       f"droplet 64^3, {steps} steps in {t:.1f} s"}
 
 
+def ensemble(prec, ndom=96, steps=200, seed=7):
+    """Batched oscillating-droplet ensemble: ndom two-component D3Q27
+    HIGHORDER periodic domains, edge L drawn from {32,48,64,96,128}, an
+    ellipsoidal droplet R0 = 0.3 L with an l = 2 perturbation of amplitude
+    U[0.02, 0.1], density ratio 50, sigma = 0.005, tau_diff = 0.1,
+    nu = 0.001, w = 5.  Aggregate MLUPS of the batched step (autotuned tile,
+    two-step captured graph) against the same domains stepped one by one."""
+    from lbfast_amd import hip
+    rng = np.random.default_rng(seed)
+    Ls = rng.choice([32, 48, 64, 96, 128], size=ndom)
+    amps = rng.uniform(0.02, 0.1, size=ndom)
+    doms = []
+    for L, a in zip(Ls, amps):
+        n = (int(L),) * 3
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, visc1=0.001, visc2=0.001,
+                        rho_r=1.0, rho_b=0.02, tau_diff=0.1, sigma=0.005, width=5.0, precision=prec)
+        f = setups.droplet(n, 0.3 * L, 5.0, [L / 2 + 0.5] * 3, 0.005, 1.0, 0.02, noise=1e-6,
+                           seed=int(rng.integers(1 << 30)), ellipse_a=float(a))
+        d = hip.Domain(p)
+        d.load(f)
+        doms.append(d)
+    nodes = int(sum(int(L) ** 3 for L in Ls))
+    # the plain kernel looped over the domains
+    for d in doms:
+        d.step(2)
+    for d in doms:
+        d.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for d in doms:
+            d.step(1)
+    for d in doms:
+        d.synchronize()
+    looped = nodes * steps / (time.perf_counter() - t0) / 1e6
+    b = hip.Batch(doms)
+    ty, tz = b.autotune(4)
+    b.capture(2)
+    b.step(2)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    batched = nodes * steps / wall / 1e6
+    b.close()
+    for d in doms:
+        d.close()
+    return {"mlups_batched": round(batched, 1), "mlups_looped": round(looped, 1),
+            "speedup": round(batched / looped, 2), "tile": [ty, tz]}, nodes
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of the fused kernel measured with rocprofv3
     counters (profiles/hbm_traffic.txt), or None."""
@@ -145,6 +196,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage legs")
     ap.add_argument("--only", default="", help="run one leg: fp32 | fp16s | bf16s | 1c")
     ap.add_argument("--tile", default="", help="ty,tz override of the fused kernel tile")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -227,6 +279,13 @@ def main():
     if not args.no_precision and not args.only:
         print("[bench] precision errors vs FP64", file=sys.stderr, flush=True)
         out["precision_error"] = precision_errors()
+    if not args.no_ensemble and not args.only:
+        print("[bench] batched ensemble", file=sys.stderr, flush=True)
+        ens = {}
+        for name, prec in (("fp32", PREC_FP32), ("fp16s", PREC_FP16S)):
+            ens[name], nodes_e = ensemble(prec)
+        out["ensemble"] = {"domains": 96, "nodes": nodes_e, "steps": 200,
+                           "workload": "96 two-component D3Q27h droplets, L in {32,48,64,96,128}", **ens}
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()
