@@ -284,8 +284,10 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     if (e != hipSuccess) return nullptr;
     return dalloc(c, bytes, &e);
   };
-  for (int b = 0; b < 2; ++b) c->h[b] = A(10 * c->N * c->tsz);
-  for (int b = 0; b < 2; ++b) c->U[b] = A(3 * c->N * c->tsz);
+  // +256 B: a 16-bit field's last element is fetched as the whole dword
+  // that holds it (the fused kernel's prefetch), which may reach 2 B past it
+  for (int b = 0; b < 2; ++b) c->h[b] = A(10 * c->N * c->tsz + 256);
+  for (int b = 0; b < 2; ++b) c->U[b] = A(3 * c->N * c->tsz + 256);
   if (p.twocomponent) {
     for (int b = 0; b < 2; ++b) c->phi[b] = A(c->N * c->asz);
     c->aux = A(7 * c->N * c->asz);

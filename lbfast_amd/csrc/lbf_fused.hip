@@ -199,51 +199,62 @@ __device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, s
   }
 }
 
-// LDS-DMA prefetch of one plane's inputs into this wave's slots (FP32 storage).
-template <bool TWO, bool INLINE>
-__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, size_t nn, uint32_t pb, uint32_t boff,
-                                        uint32_t lds0) {
+// LDS-DMA prefetch of one plane's inputs into this wave's slots.  Float
+// fields move their lane's dword; a 16-bit moment field moves the aligned
+// dword that holds the lane's element (the half is picked when read back:
+// byte (bT & 2) of the slot), which works for any lane address pattern.  Plane
+// records are rounded up to whole dwords (allocations carry padding).
+template <int PREC, bool TWO, bool INLINE>
+__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, size_t nn, uint32_t pbT, uint32_t pbA,
+                                        uint32_t bT, uint32_t bA, uint32_t lds0) {
   typedef Slots<TWO, INLINE> SL;
+  typedef typename Codec<PREC>::T T;
+  static_assert(sizeof(typename Aux<PREC>::T) == 4, "float phase-field storage");
   const DevParams& dp = a.dp;
-  const float* S = (const float*)a.min + zoff;
+  const uint32_t vT = bT & ~3u, pT = (pbT + 3u) & ~3u;
+  const T* S = (const T*)a.min + zoff;
   // slot offsets are immediates: only the wave's LDS base stays live
-  dma_dword<256u * 0>(mkrsrc(S + 0 * nn, pb), boff, lds0);
-  dma_dword<256u * 1>(mkrsrc(S + 1 * nn, pb), boff, lds0);
-  dma_dword<256u * 2>(mkrsrc(S + 2 * nn, pb), boff, lds0);
-  dma_dword<256u * 3>(mkrsrc(S + 3 * nn, pb), boff, lds0);
-  dma_dword<256u * 4>(mkrsrc(S + 4 * nn, pb), boff, lds0);
-  dma_dword<256u * 5>(mkrsrc(S + 5 * nn, pb), boff, lds0);
-  dma_dword<256u * 6>(mkrsrc(S + 6 * nn, pb), boff, lds0);
-  dma_dword<256u * 7>(mkrsrc(S + 7 * nn, pb), boff, lds0);
-  dma_dword<256u * 8>(mkrsrc(S + 8 * nn, pb), boff, lds0);
-  dma_dword<256u * 9>(mkrsrc(S + 9 * nn, pb), boff, lds0);
-  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + zoff, pb), boff, lds0);
+  dma_dword<256u * 0>(mkrsrc(S + 0 * nn, pT), vT, lds0);
+  dma_dword<256u * 1>(mkrsrc(S + 1 * nn, pT), vT, lds0);
+  dma_dword<256u * 2>(mkrsrc(S + 2 * nn, pT), vT, lds0);
+  dma_dword<256u * 3>(mkrsrc(S + 3 * nn, pT), vT, lds0);
+  dma_dword<256u * 4>(mkrsrc(S + 4 * nn, pT), vT, lds0);
+  dma_dword<256u * 5>(mkrsrc(S + 5 * nn, pT), vT, lds0);
+  dma_dword<256u * 6>(mkrsrc(S + 6 * nn, pT), vT, lds0);
+  dma_dword<256u * 7>(mkrsrc(S + 7 * nn, pT), vT, lds0);
+  dma_dword<256u * 8>(mkrsrc(S + 8 * nn, pT), vT, lds0);
+  dma_dword<256u * 9>(mkrsrc(S + 9 * nn, pT), vT, lds0);
+  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + zoff, pbA), bA, lds0);
   if constexpr (!INLINE) {
     const float* fr = (const float*)a.frc + zoff;
-    dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 1)>(mkrsrc(fr + 1 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(fr + 1 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * nn, pbA), bA, lds0);
   } else if constexpr (TWO) {
     const float* pr = (const float*)a.pre + zoff;
-    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * nn, pb), boff, lds0);
-    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * nn, pb), boff, lds0);
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * nn, pbA), bA, lds0);
+    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * nn, pbA), bA, lds0);
     if (dp.dens) {
-      const float* uo = (const float*)a.uin + zoff;
-      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * nn, pb), boff, lds0);
-      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * nn, pb), boff, lds0);
-      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * nn, pb), boff, lds0);
+      const T* uo = (const T*)a.uin + zoff;
+      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * nn, pT), vT, lds0);
+      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * nn, pT), vT, lds0);
+      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * nn, pT), vT, lds0);
     }
   }
 }
 
-template <bool TWO, bool INLINE>
-__device__ __forceinline__ void lds_raw(RawNode<PREC_FP32>& r, const float (*slot)[64], int lane, bool dens) {
+template <int PREC, bool TWO, bool INLINE>
+__device__ __forceinline__ void lds_raw(RawNode<PREC>& r, const float (*slot)[64], int lane, bool dens, uint32_t bT) {
   typedef Slots<TWO, INLINE> SL;
+  typedef typename Codec<PREC>::T T;
+  // a 16-bit element sits at byte (bT & 2) of its lane's dword
+  const uint32_t hs = sizeof(T) == 2 ? (bT & 2u) : 0u;
+  auto mom = [&](int f) { return *(const T*)((const char*)&slot[f][lane] + hs); };
 #pragma unroll
-  for (int f = 0; f < 10; ++f) r.m[f] = slot[f][lane];
+  for (int f = 0; f < 10; ++f) r.m[f] = mom(f);
   if constexpr (TWO) r.phi = slot[SL::PHI][lane];
   if constexpr (!INLINE) {
 #pragma unroll
@@ -253,7 +264,7 @@ __device__ __forceinline__ void lds_raw(RawNode<PREC_FP32>& r, const float (*slo
     for (int f = 0; f < 5; ++f) r.g[f] = slot[SL::G + f][lane];
     if (dens) {
 #pragma unroll
-      for (int f = 0; f < 3; ++f) r.uo[f] = slot[SL::UO + f][lane];
+      for (int f = 0; f < 3; ++f) r.uo[f] = mom(SL::UO + f);
     }
   }
 }
@@ -278,7 +289,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef typename C::T T;
   typedef typename Aux<PREC>::T TA;
   constexpr int NW = TY + 2;  // row waves (TY output rows + 2 halo rows)
-  constexpr bool PF = PREC == PREC_FP32;
+  constexpr bool PF = PREC == PREC_FP32 || PREC == PREC_FP16S || PREC == PREC_BF16S;
   typedef Slots<TWO, INLINE> SL;
 
   __shared__ R sUP[NW][9][64];
@@ -348,7 +359,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   int isf_next = 1;
   if constexpr (PF) {
     lds0 = lds_addr(&sPF[wave][0][0]);
-    dma_raw<TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, nn, pbT, bT, lds0);
+    dma_raw<PREC, TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, nn, pbT, pbA, bT, bA, lds0);
     if (dp.obst) isf_next = isf_load(z0 - 1);
   }
 
@@ -373,7 +384,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2, fs);
 #pragma unroll
       for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
-      lds_raw<TWO, INLINE>(cur, sPF[wave], lane, dp.dens);
+      lds_raw<PREC, TWO, INLINE>(cur, sPF[wave], lane, dp.dens, bT);
     } else {
       if (dp.obst) isf = (int8_t)isf_load(zsrc);
       load_raw<PREC, TWO, INLINE>(cur, a, zoff, fs, pbT, pbA, bT, bA);
@@ -480,7 +491,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         // every store of this iteration is issued: stream the next plane in
         lds_wait_all();
         if (zsrc < zend) {
-          dma_raw<TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, fs, pbT, bT, lds0);
+          dma_raw<PREC, TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, fs, pbT, pbA, bT, bA, lds0);
           if (dp.obst) isf_next = isf_load(zsrc + 1);
         }
       }
