@@ -1,0 +1,91 @@
+"""Input decks and runs of the Fortran host executables (lbfast_amd/fortran/bin).
+
+A deck is written with the reference's namelist layout (&simulation and
+&fluid groups closed by &end, '!' comments allowed); the host runs in its own
+working directory and writes RAW frames under output/ there.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "lbfast_amd", "fortran", "bin")
+
+
+def exe(variant: str) -> str:
+    return os.path.join(BIN, "lbfast_host_" + variant)
+
+
+def _fmt(v) -> str:
+    if isinstance(v, bool):
+        return ".true." if v else ".false."
+    if isinstance(v, (list, tuple)):
+        return ", ".join(_fmt(x) for x in v)
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def deck(simulation: dict, fluid: dict) -> str:
+    lines = ["&simulation"]
+    lines += [f"   {k}={_fmt(v)}" for k, v in simulation.items()]
+    lines += ["&end", "&fluid"]
+    lines += [f"   {k}={_fmt(v)}   ! {k}" for k, v in fluid.items()]
+    lines += ["&end", ""]
+    return "\n".join(lines)
+
+
+def run(variant: str, workdir, text: str, args=None, timeout=150) -> subprocess.CompletedProcess:
+    path = os.path.join(str(workdir), "input.inp")
+    with open(path, "w") as f:
+        f.write(text)
+    cmd = [exe(variant)] + (list(args) if args is not None else ["input.inp"])
+    return subprocess.run(cmd, cwd=str(workdir), capture_output=True, text=True, timeout=timeout)
+
+
+def header(stdout: str) -> dict:
+    """The parameter lines every run prints before touching the device."""
+    out = {}
+    m = re.search(r"lx ly lz\s+(\d+)\s+(\d+)\s+(\d+)\s+lattice\s+(\d+)\s+highorder\s+(\d+)\s+"
+                  r"twocomponent\s+(\d+)\s+precision\s+(\d+)", stdout)
+    out["n"] = tuple(int(m.group(i)) for i in (1, 2, 3))
+    out["lattice"], out["highorder"], out["twocomponent"], out["precision"] = (int(m.group(i)) for i in range(4, 8))
+    m = re.search(r"repulsive pbc\s*(.*)$", stdout, re.M)
+    vals = [int(x) for x in m.group(1).split()]
+    for k, v in zip(("densratio", "incomp", "obstacles", "buoyancy", "bounce_back", "smagorinsky", "repulsive",
+                     "pbc"), vals):
+        out[k] = v
+    m = re.search(r"tau1 omega tau2 beta kapp\s+(.*)$", stdout, re.M)
+    out["tau1"], out["omega"], out["tau2"], out["beta"], out["kapp"] = (float(x) for x in m.group(1).split())
+    return out
+
+
+def stats(stdout: str, tag: str) -> list:
+    """'TAG step N key value ...' lines as dicts."""
+    rows = []
+    for line in stdout.splitlines():
+        if line.startswith(tag + " step"):
+            tok = line.split()
+            row = {"step": int(tok[2])}
+            for k, v in zip(tok[3::2], tok[4::2]):
+                row[k] = float(v)
+            rows.append(row)
+    return rows
+
+
+def read_frame(workdir, name: str, frame: int, n, comps: int = 1) -> np.ndarray:
+    """output/out_<name>_NNNNNN.raw as float32, indexed [z, y, x] (and a
+    trailing component axis for vectors)."""
+    path = os.path.join(str(workdir), "output", f"out_{name}_{frame:06d}.raw")
+    a = np.fromfile(path, dtype=np.float32)
+    nx, ny, nz = n
+    if comps == 1:
+        return a.reshape(nz, ny, nx)
+    return a.reshape(nz, ny, nx, comps)
+
+
+def write_raw(workdir, name: str, a_zyx) -> None:
+    """A field indexed [z, y, x] as the reference's RAW input (real(4), x fastest)."""
+    np.ascontiguousarray(a_zyx, dtype=np.float32).tofile(os.path.join(str(workdir), name))
