@@ -196,7 +196,12 @@ int lbf_synchronize(lbf_ctx* ctx);
 int32_t lbf_get_step(const lbf_ctx* ctx);
 
 /* Device pointer of a state field (for zero-copy interop); NULL if absent.
- * Storage type follows the precision mode. */
+ * Storage type follows the precision mode.  Single fields (phi, lap phi,
+ * div theta n) are x-fastest nx*ny*nz arrays.  A field of a field set (the
+ * ten moments, u_old, forces, aux, repulsive locaux) is row-interleaved with
+ * the other F fields of its set: the pointer is its first row, element
+ * (i, j, k) (0-based) is at index (k*ny + j)*F*sx + i, with sx = nx rounded
+ * up to even and F = 10, 3, 3, 7, 6 respectively. */
 void* lbf_field_device_ptr(lbf_ctx* ctx, int field_id);
 
 /* Timing of the last lbf_step call's kernels, measured with HIP events on

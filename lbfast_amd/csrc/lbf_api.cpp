@@ -25,6 +25,7 @@ struct lbf_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   size_t N = 0;
+  size_t NS = 0;   // values per field of a field set: sx*ny*nz (row-interleaved sets)
   size_t tsz = 0;  // bytes of one moment value
   size_t asz = 0;  // bytes of one aux / phi / force value
   void* h[2] = {};    // 10-field sets (S/O ping-pong)
@@ -185,6 +186,7 @@ static void fill_devparams(lbf_ctx* c) {
   d.smag = p.smagorinsky != 0;
   d.rep = p.twocomponent && p.repulsive_flux;
   d.nn = (size_t)p.nx * p.ny * p.nz;
+  d.sx = set_sx(p.nx);
   // without DENSRATIO the reference's viscosity mix still reads rho_r/rho_b
   Phys<double>& q = d.pd;
   q.fx = p.fx; q.fy = p.fy; q.fz = p.fz;
@@ -273,6 +275,7 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   c->device = dev;
   fill_devparams(c);
   c->N = (size_t)p.nx * p.ny * p.nz;
+  c->NS = (size_t)set_sx(p.nx) * p.ny * p.nz;
   switch (c->prec) {
     case LBF_PREC_FP64: c->tsz = 8; c->asz = 8; break;
     case LBF_PREC_FP32: c->tsz = 4; c->asz = 4; break;
@@ -284,20 +287,19 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     if (e != hipSuccess) return nullptr;
     return dalloc(c, bytes, &e);
   };
-  // +256 B: a 16-bit field's last element is fetched as the whole dword
-  // that holds it (the fused kernel's prefetch), which may reach 2 B past it
-  for (int b = 0; b < 2; ++b) c->h[b] = A(10 * c->N * c->tsz + 256);
-  for (int b = 0; b < 2; ++b) c->U[b] = A(3 * c->N * c->tsz + 256);
+  // field sets: F rows of sx values per (y, z) row (row-interleaved)
+  for (int b = 0; b < 2; ++b) c->h[b] = A(10 * c->NS * c->tsz);
+  for (int b = 0; b < 2; ++b) c->U[b] = A(3 * c->NS * c->tsz);
   if (p.twocomponent) {
     for (int b = 0; b < 2; ++b) c->phi[b] = A(c->N * c->asz);
-    c->aux = A(7 * c->N * c->asz);
+    c->aux = A(7 * c->NS * c->asz);
     c->lap = A(c->N * c->asz);
     c->dvt = A(c->N * c->asz);
-    c->pre = A(5 * c->N * c->asz);
+    c->pre = A(5 * c->NS * c->asz);
   }
-  c->frc = A(3 * c->N * c->asz);
+  c->frc = A(3 * c->NS * c->asz);
   if (c->dp.rep) {
-    c->rep = A(6 * c->N * c->asz);
+    c->rep = A(6 * c->NS * c->asz);
     c->rep_mask = (int8_t*)A(c->N);
   }
   c->isfh = (int8_t*)A((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2));
@@ -354,30 +356,41 @@ static int field_kind(const lbf_ctx* c, int id, bool* moment) {
   return c->asz == 8 ? K_F64 : K_F32;
 }
 
-static void* fset(void* base, int f, size_t n, size_t sz) { return base ? (char*)base + (size_t)f * n * sz : nullptr; }
-
-static void* field_ptr(lbf_ctx* c, int id) {
-  const size_t N = c->N, T = c->tsz, A = c->asz;
-  if (id >= 1 && id <= 10) return fset(c->h[c->cur], id - 1, N, T);
-  if (id == LBF_FIELD_PHI) return c->phi[c->cur];
-  if (id >= 12 && id <= 14) return fset(c->U[c->ucur], id - 12, N, T);
-  if (id >= 15 && id <= 17) return fset(c->frc, id - 15, N, A);
-  if (id >= 18 && id <= 24) return fset(c->aux, id - 18, N, A);
-  if (id == LBF_FIELD_LAPPHI) return c->lap;
-  if (id == LBF_FIELD_DIVTHN) return c->dvt;
-  if (id >= 27 && id <= 36) return fset(c->h[1 - c->cur], id - 27, N, T);
-  if (id == LBF_FIELD_PHI_O) return c->phi[1 - c->cur];
-  if (id >= LBF_FIELD_LOCAUX3 && id <= LBF_FIELD_LOCAUX8) return fset(c->rep, id - LBF_FIELD_LOCAUX3, N, A);
-  return nullptr;
+// Where a field id lives: the buffer and its place in it (field f of an
+// F-field row-interleaved set, or F = 0 for a single x-fastest array).
+struct FieldRef {
+  void* base;
+  FieldLay lay;
+  size_t esz;  // bytes of one stored value
+};
+static FieldRef field_ref(lbf_ctx* c, int id) {
+  const size_t T = c->tsz, A = c->asz;
+  if (id >= 1 && id <= 10) return {c->h[c->cur], {10, id - 1}, T};
+  if (id == LBF_FIELD_PHI) return {c->phi[c->cur], {0, 0}, A};
+  if (id >= 12 && id <= 14) return {c->U[c->ucur], {3, id - 12}, T};
+  if (id >= 15 && id <= 17) return {c->frc, {3, id - 15}, A};
+  if (id >= 18 && id <= 24) return {c->aux, {7, id - 18}, A};
+  if (id == LBF_FIELD_LAPPHI) return {c->lap, {0, 0}, A};
+  if (id == LBF_FIELD_DIVTHN) return {c->dvt, {0, 0}, A};
+  if (id >= 27 && id <= 36) return {c->h[1 - c->cur], {10, id - 27}, T};
+  if (id == LBF_FIELD_PHI_O) return {c->phi[1 - c->cur], {0, 0}, A};
+  if (id >= LBF_FIELD_LOCAUX3 && id <= LBF_FIELD_LOCAUX8) return {c->rep, {6, id - LBF_FIELD_LOCAUX3}, A};
+  return {nullptr, {0, 0}, 0};
 }
 
-void* lbf_field_device_ptr(lbf_ctx* c, int id) { return c ? field_ptr(c, id) : nullptr; }
+void* lbf_field_device_ptr(lbf_ctx* c, int id) {
+  if (!c) return nullptr;
+  const FieldRef r = field_ref(c, id);
+  if (!r.base) return nullptr;
+  // a set field: its first row; its rows recur every F*sx values
+  return (char*)r.base + (size_t)r.lay.f * (size_t)c->dp.sx * r.esz;
+}
 
 int lbf_upload_field(lbf_ctx* c, int id, const void* host, int dtype) {
   if (!c || !host) return LBF_EINVAL;
   if (dtype != 0 && dtype != 1) return fail(c, LBF_EINVAL, "dtype must be 0 (float64) or 1 (float32)");
-  void* dst = field_ptr(c, id);
-  if (!dst) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
+  const FieldRef dst = field_ref(c, id);
+  if (!dst.base) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
   bool mom;
   const int kind = field_kind(c, id, &mom);
   const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
@@ -388,16 +401,16 @@ int lbf_upload_field(lbf_ctx* c, int id, const void* host, int dtype) {
     // staged in FP32; encoded as deviations by lbf_init_state
     if (!c->raw) {
       hipError_t e;
-      c->raw = (float*)dalloc(c, 10 * c->N * 4, &e);
+      c->raw = (float*)dalloc(c, 10 * c->NS * 4, &e);  // a 10-field FP32 set
       if (e != hipSuccess) return fail(c, LBF_EHIP, "raw staging alloc failed");
     }
-    HIPCHK(c, launch_convert_in(c->staging, dtype, c->raw + (size_t)(id - 1) * c->N, K_F32, c->N, 1.0f, c->stream));
+    HIPCHK(c, launch_convert_in(c->staging, dtype, c->raw, K_F32, FieldLay{10, id - 1}, c->dp, 1.0f, c->stream));
     c->raw_have[id - 1] = true;
     c->raw_pending = true;
   } else {
     float sc = 1.0f;
     if (devc && id >= 12 && id <= 14) sc = c->dp.scale[id - 11];
-    HIPCHK(c, launch_convert_in(c->staging, dtype, dst, kind, c->N, sc, c->stream));
+    HIPCHK(c, launch_convert_in(c->staging, dtype, dst.base, kind, dst.lay, c->dp, sc, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBF_OK;
@@ -434,8 +447,8 @@ int lbf_download_field(lbf_ctx* c, int id, void* host, int dtype) {
     }
     return LBF_OK;
   }
-  void* src = field_ptr(c, id);
-  if (!src) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
+  const FieldRef src = field_ref(c, id);
+  if (!src.base) return fail(c, LBF_EINVAL, "field %d not present in this configuration", id);
   bool mom;
   const int kind = field_kind(c, id, &mom);
   const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
@@ -447,7 +460,7 @@ int lbf_download_field(lbf_ctx* c, int id, void* host, int dtype) {
   } else {
     float inv = 1.0f;
     if (devc && id >= 12 && id <= 14) inv = c->dp.invscale[id - 11];
-    HIPCHK(c, launch_convert_out(src, kind, c->staging, dtype, c->N, inv, c->stream));
+    HIPCHK(c, launch_convert_out(src.base, kind, src.lay, c->staging, dtype, c->dp, inv, c->stream));
   }
   HIPCHK(c, hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -534,19 +547,19 @@ int lbf_init_state(lbf_ctx* c) {
     c->raw_pending = false;
   }
   // O := S (init_conditions_module.f90:584-587), u_old := u
-  HIPCHK(c, launch_copy(c->h[1 - c->cur], c->h[c->cur], 10 * c->N * c->tsz, c->stream));
+  HIPCHK(c, launch_copy(c->h[1 - c->cur], c->h[c->cur], 10 * c->NS * c->tsz, c->stream));
   for (int b = 0; b < 2; ++b)
-    HIPCHK(c, launch_copy(c->U[b], (char*)c->h[c->cur] + c->N * c->tsz, 3 * c->N * c->tsz, c->stream));
+    HIPCHK(c, launch_set_copy(c->U[b], FieldLay{3, 0}, c->h[c->cur], FieldLay{10, 1}, 3, (int)c->tsz, c->dp, c->stream));
   if (c->p.twocomponent) {
     HIPCHK(c, launch_copy(c->phi[1 - c->cur], c->phi[c->cur], c->N * c->asz, c->stream));
-    HIPCHK(c, launch_fill_zero(c->aux, 7 * c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->aux, 7 * c->NS * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->lap, c->N * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->dvt, c->N * c->asz, c->stream));
-    HIPCHK(c, launch_fill_zero(c->pre, 5 * c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->pre, 5 * c->NS * c->asz, c->stream));
   }
-  HIPCHK(c, launch_fill_zero(c->frc, 3 * c->N * c->asz, c->stream));
+  HIPCHK(c, launch_fill_zero(c->frc, 3 * c->NS * c->asz, c->stream));
   if (c->rep) {
-    HIPCHK(c, launch_fill_zero(c->rep, 6 * c->N * c->asz, c->stream));
+    HIPCHK(c, launch_fill_zero(c->rep, 6 * c->NS * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->rep_mask, c->N, c->stream));
   }
   // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0
@@ -995,7 +1008,7 @@ int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int3
   std::vector<void*> snap;
   auto release = [&]() { for (void* q : snap) hipFree(q); snap.clear(); };
   for (lbf_ctx* c : b->ctx) {
-    const size_t bytes[3] = {10 * c->N * c->tsz, 3 * c->N * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
+    const size_t bytes[3] = {10 * c->NS * c->tsz, 3 * c->NS * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
     void* src[3] = {c->h[c->cur], c->U[c->ucur], c->p.twocomponent ? c->phi[c->cur] : nullptr};
     for (int i = 0; i < 3; ++i) {
       void* q = nullptr;
@@ -1028,7 +1041,7 @@ int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int3
   // restore the snapshot (trial steps were even: the parities are unchanged)
   size_t k = 0;
   for (lbf_ctx* c : b->ctx) {
-    const size_t bytes[3] = {10 * c->N * c->tsz, 3 * c->N * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
+    const size_t bytes[3] = {10 * c->NS * c->tsz, 3 * c->NS * c->tsz, c->p.twocomponent ? c->N * c->asz : 0};
     void* dst[3] = {c->h[c->cur], c->U[c->ucur], c->p.twocomponent ? c->phi[c->cur] : nullptr};
     for (int i = 0; i < 3; ++i, ++k)
       if (bytes[i]) BHIPCHK(b, hipMemcpyAsync(dst[i], snap[k], bytes[i], hipMemcpyDeviceToDevice, b->stream));

@@ -24,7 +24,8 @@ struct DevParams {
   int twocomp, dens, incomp, obst, buoy;
   int lattice, highorder;
   int bb, smag, rep;     // BOUNCE_BACK (with obstacles), SMAGORINSKI, REPULSIVE_FLUX
-  size_t nn;             // nx*ny*nz: stride between the fields of a field set
+  size_t nn;             // nx*ny*nz nodes
+  int sx;                // row length of one field in a field set (nx rounded up to even)
   Phys<double> pd;
   Phys<float> pf;
   float scale[10];       // FP16S/BF16S: stored = scale * deviation
@@ -34,8 +35,25 @@ struct DevParams {
 template <> __device__ __forceinline__ const Phys<double>& DevParams::ph<double>() const { return pd; }
 template <> __device__ __forceinline__ const Phys<float>& DevParams::ph<float>() const { return pf; }
 
+// Single fields (phi, lap, div theta, masks) are x-fastest arrays of nx*ny*nz.
 __host__ __device__ inline size_t nidx(int x, int y, int z, int nx, int ny) {
   return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z);
+}
+
+// Field sets (the ten moments, u_old, forces, aux, the prep output) are
+// row-interleaved: the F field rows of one (y, z) row follow each other, each
+// sx values long, so element (x, y, z) of field f sits at
+//   ((z*ny + y)*F + f)*sx + x.
+// A wave's 64-lane segment of every field of a set then lies within one
+// F*sx run of memory instead of F arrays nx*ny*nz apart, which keeps the
+// streaming kernels' DRAM pages and translations local.  sbase() is field 0;
+// field f adds f*sx.
+__host__ __device__ inline size_t sbase(int F, int x, int y, int z, int sx, int ny) {
+  return ((size_t)z * ny + y) * (size_t)F * sx + x;
+}
+__host__ __device__ inline int set_sx(int nx) { return (nx + 1) & ~1; }
+__host__ __device__ inline size_t set_elems(int F, int nx, int ny, int nz) {
+  return (size_t)F * set_sx(nx) * ny * nz;
 }
 
 // Load through a wave-uniform base pointer and a 32-bit byte offset (selects
@@ -199,8 +217,9 @@ template <class R> struct Mom {
   R P, u, v, w, xx, yy, zz, xy, xz, yz;
 };
 
-// Field set = 10 fields, field f at base + f*fs.  `base` already includes any
-// wave-uniform plane offset; `boff` is the per-lane byte offset.
+// Ten-field moment set: field f at base + f*fs (fs = sx for the row-interleaved
+// sets).  `base` already includes any wave-uniform plane offset; `boff` is
+// the per-lane byte offset.
 template <int P>
 __device__ __forceinline__ Mom<typename Codec<P>::R> load_mom(const typename Codec<P>::T* base, size_t fs,
                                                               uint32_t boff, const DevParams& dp) {

@@ -175,26 +175,41 @@ struct Slots {
   static constexpr int N = INLINE ? (TWO ? 19 : 10) : 14;
 };
 
+// Per-lane byte offsets of one node inside a z plane of each input buffer,
+// and the planes' sizes in bytes: the 10-field moment set, the 3-field
+// u_old set, the g set (prep output, 5 fields, or forces, 3 fields) and the
+// single phase field.  Field sets are row-interleaved, so a set's field f is
+// the same lane offset from a plane base shifted by f*sx elements.
+struct FOff {
+  uint32_t m, u, g, p;
+  uint32_t pm, pu, pg, pp;
+};
+
 // Direct loads of one plane's inputs (used when the inputs are not prefetched).
+// fs = sx, the field stride of the sets.
 template <int PREC, bool TWO, bool INLINE>
-__device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, size_t zoff, size_t nn, uint32_t pbT,
-                                         uint32_t pbA, uint32_t bT, uint32_t bA) {
+__device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, int zw, size_t fs, const FOff& o) {
   typedef typename Codec<PREC>::T T;
   typedef typename Aux<PREC>::T TA;
+  constexpr int GF = INLINE ? 5 : 3;
   const DevParams& dp = a.dp;
-  const T* S = (const T*)a.min + zoff;
+  const size_t rows = (size_t)zw * dp.ny;
+  const T* S = (const T*)a.min + rows * 10 * fs;
 #pragma unroll
-  for (int f = 0; f < 10; ++f) r.m[f] = fld(S, nn, f, pbT, bT);
-  if constexpr (TWO) r.phi = fld((const TA*)a.phin + zoff, nn, 0, pbA, bA);
+  for (int f = 0; f < 10; ++f) r.m[f] = fld(S, fs, f, o.pm, o.m);
+  if constexpr (TWO) r.phi = fld((const TA*)a.phin + rows * dp.nx, fs, 0, o.pp, o.p);
   if constexpr (!INLINE) {
+    const TA* fr = (const TA*)a.frc + rows * GF * fs;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) r.g[f] = fld((const TA*)a.frc + zoff, nn, f, pbA, bA);
+    for (int f = 0; f < 3; ++f) r.g[f] = fld(fr, fs, f, o.pg, o.g);
   } else if constexpr (TWO) {
+    const TA* pr = (const TA*)a.pre + rows * GF * fs;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) r.g[f] = fld((const TA*)a.pre + zoff, nn, f, pbA, bA);
+    for (int f = 0; f < 5; ++f) r.g[f] = fld(pr, fs, f, o.pg, o.g);
     if (dp.dens) {
+      const T* uo = (const T*)a.uin + rows * 3 * fs;
 #pragma unroll
-      for (int f = 0; f < 3; ++f) r.uo[f] = fld((const T*)a.uin + zoff, nn, f, pbT, bT);
+      for (int f = 0; f < 3; ++f) r.uo[f] = fld(uo, fs, f, o.pu, o.u);
     }
   }
 }
@@ -202,46 +217,47 @@ __device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, s
 // LDS-DMA prefetch of one plane's inputs into this wave's slots.  Float
 // fields move their lane's dword; a 16-bit moment field moves the aligned
 // dword that holds the lane's element (the half is picked when read back:
-// byte (bT & 2) of the slot), which works for any lane address pattern.  Plane
-// records are rounded up to whole dwords (allocations carry padding).
+// byte (o.m & 2) of the slot; sx is even, so every 16-bit field row starts on
+// a dword and the u_old offsets share that parity).
 template <int PREC, bool TWO, bool INLINE>
-__device__ __forceinline__ void dma_raw(const FusedArgs& a, size_t zoff, size_t nn, uint32_t pbT, uint32_t pbA,
-                                        uint32_t bT, uint32_t bA, uint32_t lds0) {
+__device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, const FOff& o, uint32_t lds0) {
   typedef Slots<TWO, INLINE> SL;
   typedef typename Codec<PREC>::T T;
   static_assert(sizeof(typename Aux<PREC>::T) == 4, "float phase-field storage");
+  constexpr int GF = INLINE ? 5 : 3;
   const DevParams& dp = a.dp;
-  const uint32_t vT = bT & ~3u, pT = (pbT + 3u) & ~3u;
-  const T* S = (const T*)a.min + zoff;
+  const size_t rows = (size_t)zw * dp.ny;
+  const uint32_t vM = o.m & ~3u, vU = o.u & ~3u;
+  const T* S = (const T*)a.min + rows * 10 * fs;
   // slot offsets are immediates: only the wave's LDS base stays live
-  dma_dword<256u * 0>(mkrsrc(S + 0 * nn, pT), vT, lds0);
-  dma_dword<256u * 1>(mkrsrc(S + 1 * nn, pT), vT, lds0);
-  dma_dword<256u * 2>(mkrsrc(S + 2 * nn, pT), vT, lds0);
-  dma_dword<256u * 3>(mkrsrc(S + 3 * nn, pT), vT, lds0);
-  dma_dword<256u * 4>(mkrsrc(S + 4 * nn, pT), vT, lds0);
-  dma_dword<256u * 5>(mkrsrc(S + 5 * nn, pT), vT, lds0);
-  dma_dword<256u * 6>(mkrsrc(S + 6 * nn, pT), vT, lds0);
-  dma_dword<256u * 7>(mkrsrc(S + 7 * nn, pT), vT, lds0);
-  dma_dword<256u * 8>(mkrsrc(S + 8 * nn, pT), vT, lds0);
-  dma_dword<256u * 9>(mkrsrc(S + 9 * nn, pT), vT, lds0);
-  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + zoff, pbA), bA, lds0);
+  dma_dword<256u * 0>(mkrsrc(S + 0 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 1>(mkrsrc(S + 1 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 2>(mkrsrc(S + 2 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 3>(mkrsrc(S + 3 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 4>(mkrsrc(S + 4 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 5>(mkrsrc(S + 5 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 6>(mkrsrc(S + 6 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 7>(mkrsrc(S + 7 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 8>(mkrsrc(S + 8 * fs, o.pm), vM, lds0);
+  dma_dword<256u * 9>(mkrsrc(S + 9 * fs, o.pm), vM, lds0);
+  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + rows * dp.nx, o.pp), o.p, lds0);
   if constexpr (!INLINE) {
-    const float* fr = (const float*)a.frc + zoff;
-    dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 1)>(mkrsrc(fr + 1 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * nn, pbA), bA, lds0);
+    const float* fr = (const float*)a.frc + rows * GF * fs;
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(fr + 1 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * fs, o.pg), o.g, lds0);
   } else if constexpr (TWO) {
-    const float* pr = (const float*)a.pre + zoff;
-    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * nn, pbA), bA, lds0);
-    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * nn, pbA), bA, lds0);
+    const float* pr = (const float*)a.pre + rows * GF * fs;
+    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * fs, o.pg), o.g, lds0);
+    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * fs, o.pg), o.g, lds0);
     if (dp.dens) {
-      const T* uo = (const T*)a.uin + zoff;
-      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * nn, pT), vT, lds0);
-      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * nn, pT), vT, lds0);
-      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * nn, pT), vT, lds0);
+      const T* uo = (const T*)a.uin + rows * 3 * fs;
+      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * fs, o.pu), vU, lds0);
+      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * fs, o.pu), vU, lds0);
+      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * fs, o.pu), vU, lds0);
     }
   }
 }
@@ -282,7 +298,9 @@ __host__ __device__ inline int fused_tiles(const FusedArgs& a) {
 }
 
 // One block: tile `tile` (x fastest, then y, then z chunks) of the domain `a`.
-template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
+// OB: the domain has internal obstacles (isfluid loads, solid nodes, the
+// near-wall closures); without them that code is compiled out.
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef Codec<PREC> C;
   typedef typename C::R R;
@@ -300,7 +318,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   const DevParams& dp = a.dp;
   const Phys<R>& ph = dp.ph<R>();
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
-  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
+  const size_t nxny = (size_t)nx * ny;
+  const bool obst = OB && dp.obst != 0, bb = OB && dp.bb != 0;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tzi = tile / (ntx * nty);
@@ -329,10 +348,19 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   const bool out_node = out_row && lane < XW;
   const int xw = ((xs % nx) + nx) % nx;  // lanes past a short row wrap more than once
   const int yw = ((ys % ny) + ny) % ny;  // ys may exceed 2ny when ny < TY
-  const uint32_t inplane = (uint32_t)xw + (uint32_t)nx * (uint32_t)yw;
-  const uint32_t bT = inplane * (uint32_t)sizeof(T);
-  const uint32_t bA = inplane * (uint32_t)sizeof(TA);
-  const uint32_t pbT = (uint32_t)(nxny * sizeof(T)), pbA = (uint32_t)(nxny * sizeof(TA));
+  constexpr int GF = INLINE ? 5 : 3;     // fields of the g set: prep output or forces
+  const int sx = dp.sx;
+  const uint32_t srow = (uint32_t)sx * (uint32_t)yw;
+  FOff off;
+  off.m = ((uint32_t)xw + 10u * srow) * (uint32_t)sizeof(T);
+  off.u = ((uint32_t)xw + 3u * srow) * (uint32_t)sizeof(T);
+  off.g = ((uint32_t)xw + (uint32_t)GF * srow) * (uint32_t)sizeof(TA);
+  off.p = ((uint32_t)xw + (uint32_t)nx * (uint32_t)yw) * (uint32_t)sizeof(TA);
+  const uint32_t splane = (uint32_t)sx * (uint32_t)ny;
+  off.pm = ((10u * splane * (uint32_t)sizeof(T)) + 3u) & ~3u;
+  off.pu = ((3u * splane * (uint32_t)sizeof(T)) + 3u) & ~3u;
+  off.pg = (uint32_t)GF * splane * (uint32_t)sizeof(TA);
+  off.pp = (uint32_t)(nxny * sizeof(TA));
   // isfluid with halo: per-lane in-plane offset, per-plane uniform base
   const uint32_t bI = (uint32_t)(clampi(xs, -1, nx) + 1) + (uint32_t)(nx + 2) * (uint32_t)(clampi(ys, -1, ny) + 1);
   const size_t hplane = (size_t)(nx + 2) * (ny + 2);
@@ -342,6 +370,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   };
 
   T* Sout = (T*)a.mout;
+  const size_t plM = (size_t)10 * sx * ny;  // elements of one z plane of the moment set
 
   R lo[10], mid[10], hi[10];
 #pragma unroll
@@ -350,7 +379,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
     Mom<R> o;
     o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
     o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
-    store_mom_b<PREC>(Sout + (size_t)zo * nxny, fs, pbT, bT, o, dp);
+    store_mom_b<PREC>(Sout + (size_t)zo * plM, fs, off.pm, off.m, o, dp);
   };
 
   const R third = (R)(1.0 / 3.0), twothird = (R)(2.0 / 3.0);
@@ -359,18 +388,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   int isf_next = 1;
   if constexpr (PF) {
     lds0 = lds_addr(&sPF[wave][0][0]);
-    dma_raw<PREC, TWO, INLINE>(a, (size_t)wrapg(z0 - 1, nz) * nxny, nn, pbT, pbA, bT, bA, lds0);
-    if (dp.obst) isf_next = isf_load(z0 - 1);
+    dma_raw<PREC, TWO, INLINE>(a, wrapg(z0 - 1, nz), (size_t)sx, off, lds0);
+    if (obst) isf_next = isf_load(z0 - 1);
   }
 
   for (int zsrc = z0 - 1; zsrc <= zend; ++zsrc) {
     const int zw = wrapg(zsrc, nz);
-    const size_t zoff = (size_t)zw * nxny;
     const bool own_plane = zsrc >= z0 && zsrc < zend;
     // the field stride, opaque to the optimiser inside the loop: the ~30
     // per-field plane offsets are then rebuilt by scalar ALU each iteration
     // instead of being hoisted and spilled to vector-register lanes
-    size_t fs = nn;
+    size_t fs = (size_t)sx;
     asm volatile("" : "+s"(fs));
 
     RawNode<PREC> cur;
@@ -384,10 +412,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2, fs);
 #pragma unroll
       for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
-      lds_raw<PREC, TWO, INLINE>(cur, sPF[wave], lane, dp.dens, bT);
+      lds_raw<PREC, TWO, INLINE>(cur, sPF[wave], lane, dp.dens, off.m);
     } else {
-      if (dp.obst) isf = (int8_t)isf_load(zsrc);
-      load_raw<PREC, TWO, INLINE>(cur, a, zoff, fs, pbT, pbA, bT, bA);
+      if (obst) isf = (int8_t)isf_load(zsrc);
+      load_raw<PREC, TWO, INLINE>(cur, a, zw, fs, off);
     }
 
     // ---- 1. decode + (inline) moments stage + collision ---------------------
@@ -407,7 +435,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         m.xy += m.u * m.v; m.xz += m.u * m.w; m.yz += m.v * m.w;
       }
       R phl = 0, rinv = (R)1, Fx, Fy, Fz;
-      const bool solid = dp.obst && isf == 0;
+      const bool solid = obst && isf == 0;
       if constexpr (TWO) phl = (R)cur.phi;
       if (TWO && dp.dens) rinv = rcp(ph.rho_r * phl + ((R)1 - phl) * ph.rho_b);
       if constexpr (!INLINE) {
@@ -440,18 +468,18 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         if (out_node && own_plane && !solid) {
           if constexpr (TWO) {
             if (dp.dens) {
-              T* uo = (T*)a.uout + zoff;
-              fst(uo, fs, 0, pbT, bT, C::enc(m.u, dp.scale[1]));
-              fst(uo, fs, 1, pbT, bT, C::enc(m.v, dp.scale[2]));
-              fst(uo, fs, 2, pbT, bT, C::enc(m.w, dp.scale[3]));
+              T* uo = (T*)a.uout + (size_t)zw * ny * 3 * fs;
+              fst(uo, fs, 0, off.pu, off.u, C::enc(m.u, dp.scale[1]));
+              fst(uo, fs, 1, off.pu, off.u, C::enc(m.v, dp.scale[2]));
+              fst(uo, fs, 2, off.pu, off.u, C::enc(m.w, dp.scale[3]));
             }
             // update_phi (update_phi:88-93), u after the moments stage
             const R t = -ph.sharp_c * dvt;
             const R po = phl - m.u * (R)0.5 * (gfx) - m.v * (R)0.5 * (gfy) - m.w * (R)0.5 * (gfz) +
                          ph.tau_diff * lap + t;
-            fst((TA*)a.phiout + zoff, fs, 0, pbA, bA, (TA)po);
+            fst((TA*)a.phiout + (size_t)zw * nxny, fs, 0, off.pp, off.p, (TA)po);
           }
-          if (dp.obst && !dp.bb && isf == -1) {
+          if (obst && !bb && isf == -1) {
             // LB_int_boundary (boundary:117-212) on the post-moments state
             has_lbc = true;
             const R cssq = (R)(1.0 / 3.0);
@@ -491,12 +519,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         // every store of this iteration is issued: stream the next plane in
         lds_wait_all();
         if (zsrc < zend) {
-          dma_raw<PREC, TWO, INLINE>(a, (size_t)wrapg(zsrc + 1, nz) * nxny, fs, pbT, pbA, bT, bA, lds0);
-          if (dp.obst) isf_next = isf_load(zsrc + 1);
+          dma_raw<PREC, TWO, INLINE>(a, wrapg(zsrc + 1, nz), fs, off, lds0);
+          if (obst) isf_next = isf_load(zsrc + 1);
         }
       }
       collide<LAT, HO, R>(in, f);
-      if (dp.bb && isf == -1 && out_node && own_plane) {
+      if (bb && isf == -1 && out_node && own_plane) {
         // BOUNCE_BACK (fused_LB_kernel_bb, fused:364-382): a population whose
         // upstream node is solid is replaced by this node's own opposite
         // post-collision population.  The solid node's pushed population is
@@ -522,11 +550,21 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
 
     // ---- 2. x-halo slices through LDS -------------------------------------------
     if (halo_wave && active) {
-      const int p = side ? -1 : 1;  // left halo streams +x into lane 0, right -x into lane XW-1
+      // left halo streams +x into lane 0, right -x into lane XW-1.  The two
+      // sides are separate branches with constant register indices: a
+      // select between the sides' values is turned by the compiler into a
+      // lane-variable index and expanded into a 27-way compare chain.
+      if (side == 0) {
 #pragma unroll
-      for (int r = -1; r <= 1; ++r)
+        for (int r = -1; r <= 1; ++r)
 #pragma unroll
-        for (int q = -1; q <= 1; ++q) sHX[w][side][(q + 1) + 3 * (r + 1)] = f[cidx(p, q, r)];
+          for (int q = -1; q <= 1; ++q) sHX[w][0][(q + 1) + 3 * (r + 1)] = f[cidx(1, q, r)];
+      } else {
+#pragma unroll
+        for (int r = -1; r <= 1; ++r)
+#pragma unroll
+          for (int q = -1; q <= 1; ++q) sHX[w][1][(q + 1) + 3 * (r + 1)] = f[cidx(-1, q, r)];
+      }
     }
     __syncthreads();
 
@@ -618,14 +656,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
     }
   }
   if constexpr (PF) {
-    if (out_row && lane < XW) store_lo(zend - 1, nn);
+    if (out_row && lane < XW) store_lo(zend - 1, (size_t)sx);
   }
 }
 
 // One domain per launch: 1-D grid, XCD-aware tile order.
-template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB>
 __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
-  fused_body<LAT, HO, PREC, TWO, INLINE, TY>(a, xcd_tile(blockIdx.x, fused_tiles<TY>(a)));
+  fused_body<LAT, HO, PREC, TWO, INLINE, TY, OB>(a, xcd_tile(blockIdx.x, fused_tiles<TY>(a)));
 }
 
 // Many domains per launch: the grid covers every domain's tiles in table
@@ -640,7 +678,7 @@ __global__ void __launch_bounds__((TY + 3) * 64)
     const int mid = (lo + hi + 1) >> 1;
     if (tab[mid].tile0 <= g) lo = mid; else hi = mid - 1;
   }
-  fused_body<LAT, HO, PREC, TWO, true, TY>(tab[lo].a, g - tab[lo].tile0);
+  fused_body<LAT, HO, PREC, TWO, true, TY, false>(tab[lo].a, g - tab[lo].tile0);  // batches hold no obstacles
 }
 
 // ---------------------------------------------------------------------------
@@ -648,7 +686,8 @@ template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
 static hipError_t launch_t(const FusedArgs& a, hipStream_t s) {
   dim3 grid(fused_tiles<TY>(a));
   dim3 block((TY + 3) * 64);
-  hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY>), grid, block, 0, s, a);
+  if (a.dp.obst) hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 

@@ -4,8 +4,10 @@
 
 namespace lbf {
 
+// Field sets are row-interleaved (sbase() in lbf_common.h); single fields
+// (phi, lap, div theta, masks) are plain x-fastest arrays.
 struct FusedArgs {
-  const void* min;   // S moments: 10-field set (field f at +f*nn elements)
+  const void* min;   // S moments: 10-field set
   void* mout;        // O moments
   const void* uin;   // previous corrected velocity: 3-field set (2c DENSRATIO)
   void* uout;        // this step's corrected velocity
@@ -63,10 +65,19 @@ hipError_t launch_phase_prep(const StageArgs& a, int prec, int tz, hipStream_t s
 hipError_t launch_encode_moments(void* S, const float* raw, const DevParams& dp, int prec, hipStream_t s);
 hipError_t launch_decode_field(const void* S, int comp, void* out, int out_dtype, const DevParams& dp, int prec,
                                hipStream_t s);
-hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, size_t n, float scale,
-                             hipStream_t s);
-hipError_t launch_convert_out(const void* src, int src_kind, void* dst, int dst_dtype, size_t n, float invscale,
-                              hipStream_t s);
+// A field's place in a buffer: field f of an F-field set (F > 0), or the
+// whole buffer as one x-fastest array (F == 0).
+struct FieldLay {
+  int F, f;
+};
+// host-order array (x fastest, nx*ny*nz) <-> stored field
+hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, FieldLay dl, const DevParams& dp,
+                             float scale, hipStream_t s);
+hipError_t launch_convert_out(const void* src, int src_kind, FieldLay sl, void* dst, int dst_dtype, const DevParams& dp,
+                              float invscale, hipStream_t s);
+// nf consecutive fields between sets (element size esz bytes), node by node
+hipError_t launch_set_copy(void* dst, FieldLay dl, const void* src, FieldLay sl, int nf, int esz, const DevParams& dp,
+                           hipStream_t s);
 hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                          double* partial, int nblocks, double* out8, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);

@@ -1,5 +1,5 @@
 // lbf_stages.hip -- the reference's per-stage kernels for gfx950 (one thread
-// per node, SoA field sets, in-kernel periodic wrap), the tiled phase-field
+// per node, row-interleaved field sets, in-kernel periodic wrap), the tiled phase-field
 // prep kernel of the fast path, and the encode/decode/reduction helpers.
 //
 //   k_norm        compute_norm_interface_kernel (lb_cuda_auxfields_module.f90:19-139)
@@ -37,6 +37,8 @@ __device__ __forceinline__ bool node_of(size_t t, const DevParams& dp, int& x, i
 }
 
 #define NB(f, a, b, c) ((R)(f)[nidx(wrapg(x + (a), nx), wrapg(y + (b), ny), wrapg(z + (c), nz), nx, ny)])
+// neighbour in a field set of F fields; f points at the wanted field's offset
+#define NBS(f, F, a, b, c) ((R)(f)[sbase((F), wrapg(x + (a), nx), wrapg(y + (b), ny), wrapg(z + (c), nz), sx, ny)])
 
 // isotropic D3Q27 gradient / laplacian of a field at (0,0,0), given an
 // accessor F(a,b,c); expression order of lb_cuda_auxfields_module.f90:72-130.
@@ -92,15 +94,17 @@ __global__ void __launch_bounds__(BS) k_norm(StageArgs a) {
   const R c = F(0, 0, 0);
   const R lap = LAPL(F, c);
 #undef F
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n7 = sbase(7, x, y, z, sx, ny);
   TA* aux = (TA*)a.aux;
-  aux[n] = (TA)(gx / (mg + eps));
-  aux[n + nn] = (TA)(gy / (mg + eps));
-  aux[n + 2 * nn] = (TA)(gz / (mg + eps));
-  aux[n + 3 * nn] = (TA)mg;
-  aux[n + 4 * nn] = (TA)(c * (ONE - c) * (gx / (mg + eps)));
-  aux[n + 5 * nn] = (TA)(c * (ONE - c) * (gy / (mg + eps)));
-  aux[n + 6 * nn] = (TA)(c * (ONE - c) * (gz / (mg + eps)));
+  aux[n7] = (TA)(gx / (mg + eps));
+  aux[n7 + sx] = (TA)(gy / (mg + eps));
+  aux[n7 + 2 * sx] = (TA)(gz / (mg + eps));
+  aux[n7 + 3 * sx] = (TA)mg;
+  aux[n7 + 4 * sx] = (TA)(c * (ONE - c) * (gx / (mg + eps)));
+  aux[n7 + 5 * sx] = (TA)(c * (ONE - c) * (gy / (mg + eps)));
+  aux[n7 + 6 * sx] = (TA)(c * (ONE - c) * (gz / (mg + eps)));
   ((TA*)a.lap)[n] = (TA)lap;
 }
 
@@ -115,14 +119,14 @@ __global__ void __launch_bounds__(BS) k_div(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t nn = dp.nn;
-  const TA* ax = (const TA*)a.aux + 4 * nn;
-  const TA* ay = (const TA*)a.aux + 5 * nn;
-  const TA* az = (const TA*)a.aux + 6 * nn;
+  const int sx = dp.sx;
+  const TA* ax = (const TA*)a.aux + 4 * sx;
+  const TA* ay = (const TA*)a.aux + 5 * sx;
+  const TA* az = (const TA*)a.aux + 6 * sx;
   const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0);
-#define X(a_, b_, c_) NB(ax, a_, b_, c_)
-#define Y(a_, b_, c_) NB(ay, a_, b_, c_)
-#define Z(a_, b_, c_) NB(az, a_, b_, c_)
+#define X(a_, b_, c_) NBS(ax, 7, a_, b_, c_)
+#define Y(a_, b_, c_) NBS(ay, 7, a_, b_, c_)
+#define Z(a_, b_, c_) NBS(az, 7, a_, b_, c_)
   const R d = (DX(X) + DY(Y) + DZ(Z)) * (R)3;
 #undef X
 #undef Y
@@ -152,7 +156,8 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 
   const DevParams& dp = a.dp;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
-  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
+  const size_t nxny = (size_t)nx * ny;
+  const int sx = dp.sx;
   const int tid = threadIdx.x;
   const int ntx = (nx + PTX - 1) / PTX, nty = (ny + PTY - 1) / PTY, ntz = (nz + tz - 1) / tz;
   const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -240,13 +245,13 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 #undef X
 #undef Y
 #undef Z
-        const size_t o = nidx(xg, yg, zo, nx, ny);
+        const size_t o = sbase(5, xg, yg, zo, sx, ny);
         TA* pre = (TA*)a.pre;
         pre[o] = (TA)gfx_c;
-        pre[o + nn] = (TA)gfy_c;
-        pre[o + 2 * nn] = (TA)gfz_c;
-        pre[o + 3 * nn] = (TA)lap_c;
-        pre[o + 4 * nn] = (TA)d;
+        pre[o + sx] = (TA)gfy_c;
+        pre[o + 2 * sx] = (TA)gfz_c;
+        pre[o + 3 * sx] = (TA)lap_c;
+        pre[o + 4 * sx] = (TA)d;
       }
     }
     gfx_c = gfx_n; gfy_c = gfy_n; gfz_c = gfz_n; lap_c = lap_n;
@@ -283,7 +288,8 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 
   const DevParams& dp = a.dp;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
-  const size_t nn = dp.nn, nxny = (size_t)nx * ny;
+  const size_t nxny = (size_t)nx * ny;
+  const int sx = dp.sx;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ntx = (nx + QX - 1) / QX, nty = (ny + QY - 1) / QY;
   const int x0 = (tile % ntx) * QX, y0 = ((tile / ntx) % nty) * QY;
@@ -346,11 +352,11 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
     for (int h = 0; h < 2; ++h) {
       if (st_g[h]) {
         pre[og[h]] = (TA)hg[h][0];
-        pre[og[h] + nn] = (TA)hg[h][1];
-        pre[og[h] + 2 * nn] = (TA)hg[h][2];
-        pre[og[h] + 3 * nn] = (TA)hg[h][3];
+        pre[og[h] + sx] = (TA)hg[h][1];
+        pre[og[h] + 2 * sx] = (TA)hg[h][2];
+        pre[og[h] + 3 * sx] = (TA)hg[h][3];
       }
-      if (st_d[h]) pre[od[h] + 4 * nn] = (TA)hd[h];
+      if (st_d[h]) pre[od[h] + 4 * sx] = (TA)hd[h];
       st_g[h] = false;
       st_d[h] = false;
     }
@@ -383,7 +389,7 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
             hg[h][1] = (R)(TA)nyv * mgs;
             hg[h][2] = (R)(TA)nzv * mgs;
             hg[h][3] = LAPL(F, c);
-            og[h] = (size_t)k * nxny + (size_t)y * nx + xl;
+            og[h] = sbase(5, xl, y, k, sx, ny);
             st_g[h] = true;
           }
         }
@@ -415,7 +421,7 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
         const int y = y0 - 1 + t;
         if (out_lane && y < ny && fluid(xl, y, zo)) {
           hd[h] = d;
-          od[h] = (size_t)zo * nxny + (size_t)y * nx + xl;
+          od[h] = sbase(5, xl, y, zo, sx, ny);
           st_d[h] = true;
         }
       }
@@ -427,11 +433,11 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   for (int h = 0; h < 2; ++h) {
     if (st_g[h]) {
       pre[og[h]] = (TA)hg[h][0];
-      pre[og[h] + nn] = (TA)hg[h][1];
-      pre[og[h] + 2 * nn] = (TA)hg[h][2];
-      pre[og[h] + 3 * nn] = (TA)hg[h][3];
+      pre[og[h] + sx] = (TA)hg[h][1];
+      pre[og[h] + 2 * sx] = (TA)hg[h][2];
+      pre[og[h] + 3 * sx] = (TA)hg[h][3];
     }
-    if (st_d[h]) pre[od[h] + 4 * nn] = (TA)hd[h];
+    if (st_d[h]) pre[od[h] + 4 * sx] = (TA)hd[h];
   }
 }
 
@@ -468,7 +474,7 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   const R inv_sqrt2 = (R)0.70710678118654752440, inv_sqrt3 = (R)0.57735026918962576450;
   const R half_val = (R)0.5, third_val = (R)(1.0 / 3.0);
   const R pi = (R)3.1415926535897932384626433832795028841971;
-  const size_t nn = dp.nn;
+  const int sx = dp.sx;
   TA* ph = (TA*)a.phiS;
   const size_t n = nidx(x, y, z, nx, ny);
   R ghost = (R)ph[n];
@@ -503,16 +509,19 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   // the reference skips a donor with a zero (1-based) coordinate, i.e. one in
   // the lower halo (boundary:395); 0-based that is -1
   if (!have || dx == -1 || dy == -1 || dz == -1) return;
-  const size_t dn = nidx(wrapg(dx, nx), wrapg(dy, ny), wrapg(dz, nz), nx, ny);
+  const int dxw = wrapg(dx, nx), dyw = wrapg(dy, ny), dzw = wrapg(dz, nz);
+  const size_t dn = nidx(dxw, dyw, dzw, nx, ny);
   const R pf = (R)ph[dn];
   R gx, gy, gz;
   if (a.gf_from_pre) {
     const TA* pre = (const TA*)a.pre;
-    gx = (R)pre[dn]; gy = (R)pre[dn + nn]; gz = (R)pre[dn + 2 * nn];
+    const size_t d5 = sbase(5, dxw, dyw, dzw, sx, ny);
+    gx = (R)pre[d5]; gy = (R)pre[d5 + sx]; gz = (R)pre[d5 + 2 * sx];
   } else {
     const TA* aux = (const TA*)a.aux;
-    const R mg = (R)aux[dn + 3 * nn];
-    gx = (R)aux[dn] * mg; gy = (R)aux[dn + nn] * mg; gz = (R)aux[dn + 2 * nn] * mg;
+    const size_t d7 = sbase(7, dxw, dyw, dzw, sx, ny);
+    const R mg = (R)aux[d7 + 3 * sx];
+    gx = (R)aux[d7] * mg; gy = (R)aux[d7 + sx] * mg; gz = (R)aux[d7 + 2 * sx] * mg;
   }
   const R gdw = gx * wnx + gy * wny + gz * wnz;
   const R tx = gx - gdw * wnx, ty = gy - gdw * wny, tz = gz - gdw * wnz;
@@ -540,48 +549,51 @@ __global__ void __launch_bounds__(BS) k_moments(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n10 = sbase(10, x, y, z, sx, ny), n3 = sbase(3, x, y, z, sx, ny);
   const uint32_t zero = 0;
-  Mom<R> m = load_mom<PREC>((const T*)a.S + n, nn, zero, dp);
+  Mom<R> m = load_mom<PREC>((const T*)a.S + n10, sx, zero, dp);
   R phl = 0, gfx = 0, gfy = 0, gfz = 0, lap = 0, dvt = 0, uox = 0, uoy = 0, uoz = 0;
   R tFx, tFy, tFz;
   if (dp.twocomp) {
     const TA* aux = (const TA*)a.aux;
+    const size_t n7 = sbase(7, x, y, z, sx, ny);
     phl = (R)((const TA*)a.phiS)[n];
-    const R mg = (R)aux[n + 3 * nn];
-    gfx = (R)aux[n] * mg;
-    gfy = (R)aux[n + nn] * mg;
-    gfz = (R)aux[n + 2 * nn] * mg;
+    const R mg = (R)aux[n7 + 3 * sx];
+    gfx = (R)aux[n7] * mg;
+    gfy = (R)aux[n7 + sx] * mg;
+    gfz = (R)aux[n7 + 2 * sx] * mg;
     lap = (R)((const TA*)a.lap)[n];
     dvt = (R)((const TA*)a.dvt)[n];
     if (dp.dens) {
       const T* uo = (const T*)a.uin;
-      uox = C::dec(uo[n], dp.invscale[1]);
-      uoy = C::dec(uo[n + nn], dp.invscale[2]);
-      uoz = C::dec(uo[n + 2 * nn], dp.invscale[3]);
+      uox = C::dec(uo[n3], dp.invscale[1]);
+      uoy = C::dec(uo[n3 + sx], dp.invscale[2]);
+      uoz = C::dec(uo[n3 + 2 * sx], dp.invscale[3]);
     }
     R rfl[3] = {0, 0, 0};
     if (dp.rep) {
-      const TA* rf = (const TA*)a.rep + 3 * nn;
-      rfl[0] = (R)rf[n]; rfl[1] = (R)rf[n + nn]; rfl[2] = (R)rf[n + 2 * nn];
+      const TA* rf = (const TA*)a.rep + sbase(6, x, y, z, sx, ny) + 3 * sx;
+      rfl[0] = (R)rf[0]; rfl[1] = (R)rf[sx]; rfl[2] = (R)rf[2 * sx];
     }
     moments_stage<true, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz,
                            dp.rep ? rfl : nullptr);
   } else {
     moments_stage<false, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
   }
-  store_mom<PREC>((T*)a.S + n, nn, zero, m, dp);
+  store_mom<PREC>((T*)a.S + n10, sx, zero, m, dp);
   TA* fr = (TA*)a.frc;
-  fr[n] = (TA)tFx;
-  fr[n + nn] = (TA)tFy;
-  fr[n + 2 * nn] = (TA)tFz;
+  fr[n3] = (TA)tFx;
+  fr[n3 + sx] = (TA)tFy;
+  fr[n3 + 2 * sx] = (TA)tFz;
   // the corrected velocity is the next step's u_old (the reference reads it
   // back from this buffer once S and O have swapped)
-  const Mom<R> back = load_mom<PREC>((const T*)a.S + n, nn, zero, dp);
+  const Mom<R> back = load_mom<PREC>((const T*)a.S + n10, sx, zero, dp);
   T* uo = (T*)a.uout;
-  uo[n] = C::enc(back.u, dp.scale[1]);
-  uo[n + nn] = C::enc(back.v, dp.scale[2]);
-  uo[n + 2 * nn] = C::enc(back.w, dp.scale[3]);
+  uo[n3] = C::enc(back.u, dp.scale[1]);
+  uo[n3 + sx] = C::enc(back.v, dp.scale[2]);
+  uo[n3 + 2 * sx] = C::enc(back.w, dp.scale[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,15 +609,17 @@ __global__ void __launch_bounds__(BS) k_update_phi(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n7 = sbase(7, x, y, z, sx, ny);
   const Phys<R>& ph = dp.ph<R>();
   const TA* aux = (const TA*)a.aux;
   const R t = -ph.sharp_c * (R)((const TA*)a.dvt)[n];
-  const R mg = (R)aux[n + 3 * nn];
-  const R gx = (R)aux[n] * mg, gy = (R)aux[n + nn] * mg, gz = (R)aux[n + 2 * nn] * mg;
+  const R mg = (R)aux[n7 + 3 * sx];
+  const R gx = (R)aux[n7] * mg, gy = (R)aux[n7 + sx] * mg, gz = (R)aux[n7 + 2 * sx] * mg;
   const R phv = (R)((const TA*)a.phiS)[n];
   const R lap = (R)((const TA*)a.lap)[n];
-  const Mom<R> m = load_mom<PREC>((const T*)a.S + n, nn, 0u, dp);
+  const Mom<R> m = load_mom<PREC>((const T*)a.S + sbase(10, x, y, z, sx, ny), sx, 0u, dp);
   const R o = phv - m.u * (R)0.5 * (gx) - m.v * (R)0.5 * (gy) - m.w * (R)0.5 * (gz) + ph.tau_diff * lap + t;
   ((TA*)a.phiO)[n] = (TA)o;
 }
@@ -622,15 +636,17 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
   if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   if (isf_at(a.isfh, x, y, z, nx, ny, nz) != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n10 = sbase(10, x, y, z, sx, ny);
   const int nl = dp.lattice == 27 ? 27 : 19;
   const Phys<R>& ph = dp.ph<R>();
   const R cssq = (R)(1.0 / 3.0), invcssq = (R)3, HALF = (R)0.5, ONE = 1, TWO = 2;
   R phl = 0, rhophi = 1;
   if (dp.twocomp) phl = (R)((const TA*)a.phiS)[n];
   if (dp.twocomp && dp.dens) rhophi = ph.rho_r * phl + (ONE - phl) * ph.rho_b;
-  const Mom<R> s = load_mom<PREC>((const T*)a.S + n, nn, 0u, dp);
-  Mom<R> o = load_mom<PREC>((const T*)a.O + n, nn, 0u, dp);
+  const Mom<R> s = load_mom<PREC>((const T*)a.S + n10, sx, 0u, dp);
+  Mom<R> o = load_mom<PREC>((const T*)a.O + n10, sx, 0u, dp);
   const R pxx = s.xx - s.u * s.u, pyy = s.yy - s.v * s.v, pzz = s.zz - s.w * s.w;
   const R pxy = s.xy - s.u * s.v, pxz = s.xz - s.u * s.w, pyz = s.yz - s.v * s.w;
   R om;
@@ -667,7 +683,7 @@ __global__ void __launch_bounds__(BS) k_lb_int_bnd(StageArgs a) {
     o.zz = o.zz + f * (ez * ez - cssq);
     o.xy = o.xy + f * ex * ey; o.xz = o.xz + f * ex * ez; o.yz = o.yz + f * ey * ez;
   }
-  store_mom<PREC>((T*)a.O + n, nn, 0u, o, dp);
+  store_mom<PREC>((T*)a.O + n10, sx, 0u, o, dp);
 }
 
 // ---------------------------------------------------------------------------
@@ -696,17 +712,19 @@ __global__ void __launch_bounds__(BS) k_thinfilm_scan(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n6 = sbase(6, x, y, z, sx, ny), n7 = sbase(7, x, y, z, sx, ny);
   const Phys<R>& ph = dp.ph<R>();
   const TA* phi = (const TA*)a.phiS;
   const TA* aux = (const TA*)a.aux;
   TA* loc = (TA*)a.rep;
   a.rep_mask[n] = 0;
-  loc[n] = (TA)0; loc[n + nn] = (TA)0; loc[n + 2 * nn] = (TA)0;
+  loc[n6] = (TA)0; loc[n6 + sx] = (TA)0; loc[n6 + 2 * sx] = (TA)0;
   R q = fmin(fmax((R)phi[n], (R)0), (R)1);
   q = q * ((R)1 - q);
   if (q < ph.q_th) return;
-  const R nix = (R)aux[n], niy = (R)aux[n + nn], niz = (R)aux[n + 2 * nn];
+  const R nix = (R)aux[n7], niy = (R)aux[n7 + sx], niz = (R)aux[n7 + 2 * sx];
   R best_r2 = (R)1.0e300, best_face = (R)-1;
   int bi = 0, bj = 0, bk = 0;
   bool found = false;
@@ -719,11 +737,11 @@ __global__ void __launch_bounds__(BS) k_thinfilm_scan(StageArgs a) {
         const int8_t v = isf_at(a.isfh, xi, yj, zk, nx, ny, nz);
         if (v != 1 && v != -1) continue;
         const R r2 = (R)(di * di + dj * dj + dk * dk);
-        const size_t m = nidx(xi, yj, zk, nx, ny);
+        const size_t m = nidx(xi, yj, zk, nx, ny), m7 = sbase(7, xi, yj, zk, sx, ny);
         R qn = fmin(fmax((R)phi[m], (R)0), (R)1);
         qn = qn * ((R)1 - qn);
         if (qn < ph.q_th || fabs(qn - q) > (R)0.1 * fmax(q, (R)1.0e-12)) continue;
-        const R dotn = nix * (R)aux[m] + niy * (R)aux[m + nn] + niz * (R)aux[m + 2 * nn];
+        const R dotn = nix * (R)aux[m7] + niy * (R)aux[m7 + sx] + niz * (R)aux[m7 + 2 * sx];
         if (dotn > ph.cos_opp) continue;
         const R face = (R)0.5 * ((R)1 - dotn);
         if (r2 < best_r2 - (R)1.0e-14) {
@@ -734,7 +752,7 @@ __global__ void __launch_bounds__(BS) k_thinfilm_scan(StageArgs a) {
       }
   if (found) {
     // the reference stores 1-based local indices
-    loc[n] = (TA)(bi + 1); loc[n + nn] = (TA)(bj + 1); loc[n + 2 * nn] = (TA)(bk + 1);
+    loc[n6] = (TA)(bi + 1); loc[n6 + sx] = (TA)(bj + 1); loc[n6 + 2 * sx] = (TA)(bk + 1);
     a.rep_mask[n] = 1;
   }
 }
@@ -750,19 +768,21 @@ __global__ void __launch_bounds__(BS) k_repulsive_flux(StageArgs a) {
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const int8_t isf = isf_at(a.isfh, x, y, z, nx, ny, nz);
   if (isf != 1 && isf != -1) return;
-  const size_t n = nidx(x, y, z, nx, ny), nn = dp.nn;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n6 = sbase(6, x, y, z, sx, ny), n7 = sbase(7, x, y, z, sx, ny);
   const Phys<R>& ph = dp.ph<R>();
   const TA* phi = (const TA*)a.phiS;
   const TA* aux = (const TA*)a.aux;
   TA* loc = (TA*)a.rep;
-  TA* flux = loc + 3 * nn;
-  flux[n] = (TA)0; flux[n + nn] = (TA)0; flux[n + 2 * nn] = (TA)0;
+  TA* flux = loc + 3 * sx;
+  flux[n6] = (TA)0; flux[n6 + sx] = (TA)0; flux[n6 + 2 * sx] = (TA)0;
   if (a.rep_mask[n] != 1) return;
   const R eps = (R)1.0e-9;
   const R phl = (R)phi[n];
   const R q1 = phl * ((R)1 - phl);
   if (q1 <= eps) return;
-  const int pi = (int)(R)loc[n] - 1, pj = (int)(R)loc[n + nn] - 1, pk = (int)(R)loc[n + 2 * nn] - 1;
+  const int pi = (int)(R)loc[n6] - 1, pj = (int)(R)loc[n6 + sx] - 1, pk = (int)(R)loc[n6 + 2 * sx] - 1;
   R dx = (R)(pi - x), dy = (R)(pj - y), dz = (R)(pk - z);
   const R r = sqrt(dx * dx + dy * dy + dz * dz);
   if (r <= eps) return;
@@ -770,9 +790,9 @@ __global__ void __launch_bounds__(BS) k_repulsive_flux(StageArgs a) {
   dx = dx * rinv; dy = dy * rinv; dz = dz * rinv;
   int xi = pi, yj = pj, zk = pk;
   if (!wrap_win(xi, yj, zk, dp)) return;
-  const size_t m = nidx(xi, yj, zk, nx, ny);
-  const R nx1 = (R)aux[n], ny1 = (R)aux[n + nn], nz1 = (R)aux[n + 2 * nn];
-  const R nx2 = (R)aux[m], ny2 = (R)aux[m + nn], nz2 = (R)aux[m + 2 * nn];
+  const size_t m = nidx(xi, yj, zk, nx, ny), m7 = sbase(7, xi, yj, zk, sx, ny);
+  const R nx1 = (R)aux[n7], ny1 = (R)aux[n7 + sx], nz1 = (R)aux[n7 + 2 * sx];
+  const R nx2 = (R)aux[m7], ny2 = (R)aux[m7 + sx], nz2 = (R)aux[m7 + 2 * sx];
   const R face = fmax((R)0, -(nx1 * nx2 + ny1 * ny2 + nz1 * nz2));
   if (face <= eps) return;
   R nsx = nx1 - nx2, nsy = ny1 - ny2, nsz = nz1 - nz2;
@@ -799,7 +819,7 @@ __global__ void __launch_bounds__(BS) k_repulsive_flux(StageArgs a) {
   const R fz = ph.a_rep * wdth * qcl * face * nsz;
   const R cap = (R)1.5 * (fabs(fx) + fabs(fy) + fabs(fz));
   const R scales = fmin((R)1.0, phl / fmax(cap, (R)1.0e-9));
-  flux[n] = (TA)(fx * scales); flux[n + nn] = (TA)(fy * scales); flux[n + 2 * nn] = (TA)(fz * scales);
+  flux[n6] = (TA)(fx * scales); flux[n6 + sx] = (TA)(fy * scales); flux[n6 + 2 * sx] = (TA)(fz * scales);
 }
 
 // ---------------------------------------------------------------------------
@@ -878,13 +898,15 @@ __global__ void __launch_bounds__(BS) k_encode(StageArgs a, const float* raw) {
   typedef Codec<PREC> C;
   typedef typename C::R R;
   typedef typename C::T T;
-  const size_t n = (size_t)blockIdx.x * BS + threadIdx.x, nn = a.dp.nn;
-  if (n >= nn) return;
+  int x, y, z;
+  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, a.dp, x, y, z)) return;
+  const int sx = a.dp.sx;
+  const size_t n = sbase(10, x, y, z, sx, a.dp.ny);  // raw is a 10-field FP32 set
   Mom<R> m;
-  m.P = raw[n]; m.u = raw[n + nn]; m.v = raw[n + 2 * nn]; m.w = raw[n + 3 * nn];
-  m.xx = raw[n + 4 * nn]; m.yy = raw[n + 5 * nn]; m.zz = raw[n + 6 * nn];
-  m.xy = raw[n + 7 * nn]; m.xz = raw[n + 8 * nn]; m.yz = raw[n + 9 * nn];
-  store_mom<PREC>((T*)a.S + n, nn, 0u, m, a.dp);
+  m.P = raw[n]; m.u = raw[n + sx]; m.v = raw[n + 2 * sx]; m.w = raw[n + 3 * sx];
+  m.xx = raw[n + 4 * sx]; m.yy = raw[n + 5 * sx]; m.zz = raw[n + 6 * sx];
+  m.xy = raw[n + 7 * sx]; m.xz = raw[n + 8 * sx]; m.yz = raw[n + 9 * sx];
+  store_mom<PREC>((T*)a.S + n, sx, 0u, m, a.dp);
 }
 
 hipError_t launch_encode_moments(void* S, const float* raw, const DevParams& dp, int prec, hipStream_t s) {
@@ -907,8 +929,9 @@ __global__ void __launch_bounds__(BS) k_decode(StageArgs a, int comp, O* out) {
   typedef typename C::R R;
   typedef typename C::T T;
   const size_t n = (size_t)blockIdx.x * BS + threadIdx.x;
-  if (n >= a.dp.nn) return;
-  const Mom<R> m = load_mom<PREC>((const T*)a.S + n, a.dp.nn, 0u, a.dp);
+  int x, y, z;
+  if (!node_of(n, a.dp, x, y, z)) return;
+  const Mom<R> m = load_mom<PREC>((const T*)a.S + sbase(10, x, y, z, a.dp.sx, a.dp.ny), a.dp.sx, 0u, a.dp);
   const R v[10] = {m.P, m.u, m.v, m.w, m.xx, m.yy, m.zz, m.xy, m.xz, m.yz};
   R r = 0;
 #pragma unroll
@@ -959,23 +982,51 @@ __device__ __forceinline__ void kstore(void* p, int kind, size_t i, double v, fl
     default: ((__hip_bfloat16*)p)[i] = __float2bfloat16((float)v * sc); break;
   }
 }
-__global__ void __launch_bounds__(BS) k_convert(const void* src, int sk, void* dst, int dk, size_t n, float inv,
-                                                float sc) {
+// element index of host-order node i in a field of layout l
+__device__ __forceinline__ size_t lay_at(FieldLay l, size_t i, int nx, int sx) {
+  if (l.F == 0) return i;
+  const size_t row = i / (size_t)nx;
+  return (row * l.F + l.f) * (size_t)sx + (i - row * (size_t)nx);
+}
+__global__ void __launch_bounds__(BS) k_convert(const void* src, int sk, FieldLay sl, void* dst, int dk, FieldLay dl,
+                                                size_t n, int nx, int sx, float inv, float sc) {
   const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  kstore(dst, dk, i, kload(src, sk, i, inv), sc);
+  kstore(dst, dk, lay_at(dl, i, nx, sx), kload(src, sk, lay_at(sl, i, nx, sx), inv), sc);
 }
 
-hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, size_t n, float scale,
-                             hipStream_t s) {
-  const dim3 grid((unsigned)((n + BS - 1) / BS)), block(BS);
-  hipLaunchKernelGGL(k_convert, grid, block, 0, s, src, src_dtype, dst, dst_kind, n, 1.0f, scale);
+hipError_t launch_convert_in(const void* src, int src_dtype, void* dst, int dst_kind, FieldLay dl, const DevParams& dp,
+                             float scale, hipStream_t s) {
+  const dim3 grid((unsigned)((dp.nn + BS - 1) / BS)), block(BS);
+  hipLaunchKernelGGL(k_convert, grid, block, 0, s, src, src_dtype, FieldLay{0, 0}, dst, dst_kind, dl, dp.nn, dp.nx,
+                     dp.sx, 1.0f, scale);
   return hipGetLastError();
 }
-hipError_t launch_convert_out(const void* src, int src_kind, void* dst, int dst_dtype, size_t n, float invscale,
-                              hipStream_t s) {
-  const dim3 grid((unsigned)((n + BS - 1) / BS)), block(BS);
-  hipLaunchKernelGGL(k_convert, grid, block, 0, s, src, src_kind, dst, dst_dtype, n, invscale, 1.0f);
+hipError_t launch_convert_out(const void* src, int src_kind, FieldLay sl, void* dst, int dst_dtype, const DevParams& dp,
+                              float invscale, hipStream_t s) {
+  const dim3 grid((unsigned)((dp.nn + BS - 1) / BS)), block(BS);
+  hipLaunchKernelGGL(k_convert, grid, block, 0, s, src, src_kind, sl, dst, dst_dtype, FieldLay{0, 0}, dp.nn, dp.nx,
+                     dp.sx, invscale, 1.0f);
+  return hipGetLastError();
+}
+
+template <class E>
+__global__ void __launch_bounds__(BS) k_set_copy(E* dst, FieldLay dl, const E* src, FieldLay sl, int nf, size_t n,
+                                                 int nx, int sx) {
+  const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < nf; ++k)
+    dst[lay_at(FieldLay{dl.F, dl.f + k}, i, nx, sx)] = src[lay_at(FieldLay{sl.F, sl.f + k}, i, nx, sx)];
+}
+hipError_t launch_set_copy(void* dst, FieldLay dl, const void* src, FieldLay sl, int nf, int esz, const DevParams& dp,
+                           hipStream_t s) {
+  const dim3 grid((unsigned)((dp.nn + BS - 1) / BS)), block(BS);
+  switch (esz) {
+    case 2: hipLaunchKernelGGL(k_set_copy<uint16_t>, grid, block, 0, s, (uint16_t*)dst, dl, (const uint16_t*)src, sl, nf, dp.nn, dp.nx, dp.sx); break;
+    case 4: hipLaunchKernelGGL(k_set_copy<uint32_t>, grid, block, 0, s, (uint32_t*)dst, dl, (const uint32_t*)src, sl, nf, dp.nn, dp.nx, dp.sx); break;
+    case 8: hipLaunchKernelGGL(k_set_copy<uint64_t>, grid, block, 0, s, (uint64_t*)dst, dl, (const uint64_t*)src, sl, nf, dp.nn, dp.nx, dp.sx); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1010,7 +1061,7 @@ __global__ void __launch_bounds__(BS) k_reduce_partial(StageArgs a, double* part
     node_of(t, dp, x, y, z);
     const int8_t isf = isf_at(a.isfh, x, y, z, dp.nx, dp.ny, dp.nz);
     if (isf != 1 && isf != -1) continue;
-    const Mom<R> m = load_mom<PREC>((const T*)a.S + t, dp.nn, 0u, dp);
+    const Mom<R> m = load_mom<PREC>((const T*)a.S + sbase(10, x, y, z, dp.sx, dp.ny), dp.sx, 0u, dp);
     double ph = 0, rho = 1;
     if (dp.twocomp) {
       ph = (double)((const TA*)a.phiS)[t];
