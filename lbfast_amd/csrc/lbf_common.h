@@ -168,6 +168,20 @@ __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 // a small deviation from the rest state inside the 16-bit normal range.
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_MIXED = 2, PREC_FP16S = 3, PREC_BF16S = 4 };
 
+// Stochastic rounding of the 16-bit codecs.  Round-to-nearest drops every
+// per-step change smaller than half a 16-bit ulp (11 significant bits), so
+// a slowly driven flow stalls and drifts; rounding up with probability equal
+// to the dropped fraction makes each store unbiased in expectation.  The
+// random bits of a field are the low bits of another field of the same node
+// (its FP32 value, below the 16-bit significand: chaotic from step to step)
+// XOR a per-node, per-plane seed from a 32-bit integer hash (the murmur3
+// finaliser).  No state is kept between launches, and replays of a captured
+// graph draw new bits because the values change from step to step.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+  return h;
+}
+
 template <int P> struct Codec;
 template <> struct Codec<PREC_FP64> {
   typedef double T; typedef double R;
@@ -196,6 +210,17 @@ template <> struct Codec<PREC_FP16S> {
   static constexpr bool dev = true;
   __device__ static R dec(T v, float inv) { return __half2float(v) * inv; }
   __device__ static T enc(R v, float s) { return __float2half_rn(v * s); }
+  // stochastic rounding of the scaled value x: x moved by a uniform offset in
+  // [-1/2, 1/2) half ulp (13 random bits below the half significand, in the
+  // normal range where every stored scaled deviation sits), then rounded to
+  // nearest, which rounds up with probability equal to the dropped fraction
+  __device__ static T srx(float x, uint32_t r) {
+    // the offset acts on the magnitude bits, clamped at zero (|x| below
+    // 2^-137 rounds to 0 either way)
+    const uint32_t b = __float_as_uint(x);
+    const int m = (int)(b & 0x7FFFFFFFu) + (int)(r & 0x1FFFu) - 0x1000;
+    return __float2half_rn(__uint_as_float((b & 0x80000000u) | (uint32_t)(m > 0 ? m : 0)));
+  }
   __device__ static R round(R v) { return v; }
 };
 template <> struct Codec<PREC_BF16S> {
@@ -203,6 +228,11 @@ template <> struct Codec<PREC_BF16S> {
   static constexpr bool dev = true;
   __device__ static R dec(T v, float inv) { return __bfloat162float(v) * inv; }
   __device__ static T enc(R v, float s) { return __float2bfloat16(v * s); }
+  // stochastic rounding of the scaled value x: 16 random bits added below the
+  // bfloat16 significand, then truncation
+  __device__ static T srx(float x, uint32_t r) {
+    return __builtin_bit_cast(T, (unsigned short)((__float_as_uint(x) + (r & 0xFFFFu)) >> 16));
+  }
   __device__ static R round(R v) { return v; }
 };
 
@@ -290,22 +320,33 @@ __device__ __forceinline__ Mom<typename Codec<P>::R> load_mom_b(const typename C
   return o;
 }
 
+// Store of the ten moments; the 16-bit codecs round stochastically with the
+// per-node, per-plane seed rnd.
 template <int P>
 __device__ __forceinline__ void store_mom_b(typename Codec<P>::T* plane0, size_t fs, uint32_t pb, uint32_t boff,
-                                            const Mom<typename Codec<P>::R>& o, const DevParams& dp) {
+                                            const Mom<typename Codec<P>::R>& o, const DevParams& dp,
+                                            uint32_t rnd) {
   typedef Codec<P> C; typedef typename C::R R; typedef typename C::T T;
-  fst(plane0, fs, 0, pb, boff, C::enc(o.P, dp.scale[0]));
-  if (C::dev) {
-    const T tu = C::enc(o.u, dp.scale[1]), tv = C::enc(o.v, dp.scale[2]), tw = C::enc(o.w, dp.scale[3]);
+  if constexpr (C::dev) {
+    // each field's random bits: the FP32 bits of the next field of the same
+    // group (cyclic), XOR the seed
+    auto sr = [&](float x, float xn) { return C::srx(x, __float_as_uint(xn) ^ rnd); };
+    const float x0 = o.P * dp.scale[0], x1 = o.u * dp.scale[1], x2 = o.v * dp.scale[2], x3 = o.w * dp.scale[3];
+    fst(plane0, fs, 0, pb, boff, sr(x0, x1));
+    const T tu = sr(x1, x2), tv = sr(x2, x3), tw = sr(x3, x0);
     const R ur = C::dec(tu, dp.invscale[1]), vr = C::dec(tv, dp.invscale[2]), wr = C::dec(tw, dp.invscale[3]);
     fst(plane0, fs, 1, pb, boff, tu); fst(plane0, fs, 2, pb, boff, tv); fst(plane0, fs, 3, pb, boff, tw);
-    fst(plane0, fs, 4, pb, boff, C::enc(o.xx - ur * ur, dp.scale[4]));
-    fst(plane0, fs, 5, pb, boff, C::enc(o.yy - vr * vr, dp.scale[5]));
-    fst(plane0, fs, 6, pb, boff, C::enc(o.zz - wr * wr, dp.scale[6]));
-    fst(plane0, fs, 7, pb, boff, C::enc(o.xy - ur * vr, dp.scale[7]));
-    fst(plane0, fs, 8, pb, boff, C::enc(o.xz - ur * wr, dp.scale[8]));
-    fst(plane0, fs, 9, pb, boff, C::enc(o.yz - vr * wr, dp.scale[9]));
+    const float x4 = (o.xx - ur * ur) * dp.scale[4], x5 = (o.yy - vr * vr) * dp.scale[5];
+    const float x6 = (o.zz - wr * wr) * dp.scale[6], x7 = (o.xy - ur * vr) * dp.scale[7];
+    const float x8 = (o.xz - ur * wr) * dp.scale[8], x9 = (o.yz - vr * wr) * dp.scale[9];
+    fst(plane0, fs, 4, pb, boff, sr(x4, x5));
+    fst(plane0, fs, 5, pb, boff, sr(x5, x6));
+    fst(plane0, fs, 6, pb, boff, sr(x6, x7));
+    fst(plane0, fs, 7, pb, boff, sr(x7, x8));
+    fst(plane0, fs, 8, pb, boff, sr(x8, x9));
+    fst(plane0, fs, 9, pb, boff, sr(x9, x4));
   } else {
+    fst(plane0, fs, 0, pb, boff, C::enc(o.P, 0.f));
     fst(plane0, fs, 1, pb, boff, C::enc(o.u, 0.f)); fst(plane0, fs, 2, pb, boff, C::enc(o.v, 0.f));
     fst(plane0, fs, 3, pb, boff, C::enc(o.w, 0.f));
     fst(plane0, fs, 4, pb, boff, C::enc(o.xx, 0.f)); fst(plane0, fs, 5, pb, boff, C::enc(o.yy, 0.f));

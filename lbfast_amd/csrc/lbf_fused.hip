@@ -371,6 +371,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
 
   T* Sout = (T*)a.mout;
   const size_t plM = (size_t)10 * sx * ny;  // elements of one z plane of the moment set
+  // 16-bit codecs: per-node seed of the stochastic rounding
+  const uint32_t nseed = C::dev ? mix32((uint32_t)xw * 0x9E3779B9u ^ (uint32_t)(yw + 1) * 0x85EBCA6Bu) : 0u;
 
   R lo[10], mid[10], hi[10];
 #pragma unroll
@@ -379,7 +381,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
     Mom<R> o;
     o.P = lo[0]; o.u = lo[1]; o.v = lo[2]; o.w = lo[3];
     o.xx = lo[4]; o.yy = lo[5]; o.zz = lo[6]; o.xy = lo[7]; o.xz = lo[8]; o.yz = lo[9];
-    store_mom_b<PREC>(Sout + (size_t)zo * plM, fs, off.pm, off.m, o, dp);
+    store_mom_b<PREC>(Sout + (size_t)zo * plM, fs, off.pm, off.m, o, dp, nseed ^ (uint32_t)zo * 0xC2B2AE35u);
   };
 
   const R third = (R)(1.0 / 3.0), twothird = (R)(2.0 / 3.0);
@@ -469,9 +471,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
           if constexpr (TWO) {
             if (dp.dens) {
               T* uo = (T*)a.uout + (size_t)zw * ny * 3 * fs;
-              fst(uo, fs, 0, off.pu, off.u, C::enc(m.u, dp.scale[1]));
-              fst(uo, fs, 1, off.pu, off.u, C::enc(m.v, dp.scale[2]));
-              fst(uo, fs, 2, off.pu, off.u, C::enc(m.w, dp.scale[3]));
+              T eu = C::enc(m.u, dp.scale[1]), ev = C::enc(m.v, dp.scale[2]), ew = C::enc(m.w, dp.scale[3]);
+              if constexpr (C::dev) {
+                const uint32_t st = nseed ^ (uint32_t)zw * 0x27D4EB2Fu;
+                const float xu = m.u * dp.scale[1], xv = m.v * dp.scale[2], xw_ = m.w * dp.scale[3];
+                eu = C::srx(xu, __float_as_uint(xv) ^ st);
+                ev = C::srx(xv, __float_as_uint(xw_) ^ st);
+                ew = C::srx(xw_, __float_as_uint(xu) ^ st);
+              }
+              fst(uo, fs, 0, off.pu, off.u, eu);
+              fst(uo, fs, 1, off.pu, off.u, ev);
+              fst(uo, fs, 2, off.pu, off.u, ew);
             }
             // update_phi (update_phi:88-93), u after the moments stage
             const R t = -ph.sharp_c * dvt;

@@ -7,15 +7,23 @@
   nearest even, computed by bit manipulation).
 * Time steps stay within stated error bounds of the FP64 run of the same
   problem (velocity relative L2 after 50 steps of a two-component D3Q27h
-  droplet: FP16S < 1e-2, BF16S < 2e-1; phase field < 1e-4), including sizes
-  whose planes hold an odd number of 16-bit values (the fused kernel's
-  prefetch fetches the aligned dword around each element).
+  droplet: FP16S < 3e-3, BF16S < 3e-2; phase field < 1e-4), including sizes
+  whose planes hold an odd number of 16-bit values.
+* A reduced two-phase Poiseuille channel with internal obstacles (the
+  reference's porous-channel workload at 64x48x32: walls in y, 25 % spheres,
+  dense lower 40 % at density ratio 10, body force 1e-6) after 2000 steps:
+  velocity relative L2 against FP64 over the fluid nodes FP32 < 2e-4,
+  FP16S < 1e-3, BF16S < 1e-2.  The time step rounds the 16-bit stores
+  stochastically; with round-to-nearest the FP16S error of this run grows to
+  5e-3 by 5000 steps as sub-ulp velocity increments are dropped.
 """
 import numpy as np
 import pytest
 
 from lbfast_amd import hip, setups
-from lbfast_amd.params import make_params, PREC_FP64, PREC_FP16S, PREC_BF16S, HFIELDS, HFIELD_NAMES, FIELD_PHI
+from lbfast_amd.params import (make_params, PREC_FP64, PREC_FP32, PREC_FP16S, PREC_BF16S, HFIELDS, HFIELD_NAMES,
+                               FIELD_PHI)
+from tests import cases
 
 pytestmark = pytest.mark.gpu
 
@@ -76,7 +84,7 @@ def _droplet(n, prec):
 
 
 @pytest.mark.parametrize("n", [(32, 32, 32), (33, 17, 21)])
-@pytest.mark.parametrize("prec,tol_u", [(PREC_FP16S, 1e-2), (PREC_BF16S, 2e-1)])
+@pytest.mark.parametrize("prec,tol_u", [(PREC_FP16S, 3e-3), (PREC_BF16S, 3e-2)])
 def test_sixteen_bit_steps_against_fp64(n, prec, tol_u):
     steps = 50
     print(f"[precision] prec={prec} n={n} steps={steps}", flush=True)
@@ -95,3 +103,31 @@ def test_sixteen_bit_steps_against_fp64(n, prec, tol_u):
     assert np.all(np.isfinite(u16)) and np.all(np.isfinite(P16))
     assert eu < tol_u
     assert ephi < 1e-4
+
+
+def test_porous_channel_against_fp64():
+    n, steps = (64, 48, 32), 2000
+    raw = setups.random_spheres(n, fill=0.25, rmin=3.0, rmax=6.0, seed=11, margin=2)
+    res = {}
+    for prec in (PREC_FP64, PREC_FP32, PREC_FP16S, PREC_BF16S):
+        print(f"[precision] porous channel prec={prec} n={n} steps={steps}", flush=True)
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, internal_obstacles=True,
+                        pbc=(1, 0, 1), visc1=0.05, visc2=0.05, rho_r=10.0, rho_b=1.0, tau_diff=0.1, sigma=0.005,
+                        width=4.0, fx=1e-6, precision=prec)
+        f = setups.two_layer_channel(n, raw, frac=0.4, width=4.0, rho_r=10.0, rho_b=1.0, axis=1)
+        f, isf = cases.prepare(p, f, raw, hip.classify_isfluid)
+        d = hip.Domain(p)
+        d.load(f, isf)
+        d.step(steps)
+        res[prec] = (np.stack([d.download(i) for i in (2, 3, 4)]), d.download(FIELD_PHI), isf)
+        d.close()
+    u64, phi64, isf = res[PREC_FP64]
+    fl = isf != 0
+    for prec, tol in ((PREC_FP32, 2e-4), (PREC_FP16S, 1e-3), (PREC_BF16S, 1e-2)):
+        u, phi, _ = res[prec]
+        eu = np.linalg.norm((u - u64)[:, fl]) / np.linalg.norm(u64[:, fl])
+        ephi = np.linalg.norm((phi - phi64)[fl]) / np.linalg.norm(phi64[fl])
+        print(f"[precision]   prec={prec}: u {eu:.2e} phi {ephi:.2e}", flush=True)
+        assert np.all(np.isfinite(u))
+        assert eu < tol, (prec, eu)
+        assert ephi < 1e-3, (prec, ephi)
