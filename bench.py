@@ -262,7 +262,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"laplace_droplet_2c_d3q27h_densratio_{args.n}^3", "lattice": "D3Q27 HIGHORDER",
-                   "components": 2, "densratio": 100, "nodes": nodes, "storage": "fp32 SoA",
+                   "components": 2, "densratio": 100, "nodes": nodes, "storage": "fp32, row-interleaved field sets",
                    "baseline": "1x A100 2c 512^3 D3Q27h SP 800 MLUPS (README.md:405)"},
         "roofline": {"bound": "hbm", "achieved": round(main_leg["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_leg["achieved_gbs"] / HBM_PEAK_GBS, 4),
