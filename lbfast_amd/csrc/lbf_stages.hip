@@ -263,19 +263,26 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 //
 // Same stencils and expression order as k_norm / k_div
 // (lb_cuda_auxfields_module.f90:72-130, 1110-1160), evaluated from LDS.  A
-// block of QW waves owns a tile of 64 lanes in x (x0-2 .. x0+61; outputs on
-// lanes 2..61) by QY + 4 rows of phi and marches z.  phi planes stream into a
-// ring of four LDS planes by LDS DMA three planes ahead; theta planes live in
-// a ring of three.  Iteration k computes grad/lap/theta of plane k and div
-// theta of plane k-1.  Every in-plane index is wrapped once per lane/row,
-// not per element, and isfluid is read only with internal obstacles.
-constexpr int QX = 60, QY = 14, QW = 8;
+// block owns a 64-column tile (outputs x0 .. x0+63, aligned with the field
+// rows, so every store is a whole-row segment) by QY rows and marches z.
+// Its QW row waves hold two theta rows each; one more wave computes theta on
+// the two edge columns x0-1 and x0+64 that the tile's divergence needs.
+// phi rows are kept in LDS with a 2-column halo each side (column c holds
+// x0-2+c, wrapped): the row's 64 lanes and two 2-lane transfers for the
+// halo columns stream in by LDS DMA three planes ahead, into a ring of four
+// planes; theta lives in a ring of three planes of 66 columns (x0-1 ..
+// x0+64).  Iteration k computes grad/lap/theta of plane k and div theta of
+// plane k-1.  isfluid is read only with internal obstacles.
+constexpr int QY = 12, QW = 7;
 constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
 constexpr int QT = QY + 2;                 // theta rows (1-row halo)
-constexpr int QDMA = (QR + QW - 1) / QW;   // phi rows each wave streams per plane
+constexpr int QC = 64 + 4;                 // phi columns x0-2 .. x0+65
+constexpr int TC = 64 + 2;                 // theta columns x0-1 .. x0+64
+constexpr int QDMA = (QR + QW - 1) / QW;   // phi rows each row wave streams per plane
+constexpr int QNT = (QW + 1) * 64;         // threads: row waves + the edge-column wave
 
 __host__ __device__ inline int prep_tiles_z(const DevParams& dp, int tz) {
-  return ((dp.nx + QX - 1) / QX) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
+  return ((dp.nx + 63) / 64) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
 }
 
 template <int PREC>
@@ -283,44 +290,72 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   typedef typename Codec<PREC>::R R;
   typedef typename Aux<PREC>::T TA;
   static_assert(sizeof(TA) == 4, "float phase-field storage");
-  __shared__ float sphi[4][QR + 1][64];  // row QR absorbs the padding transfers
-  __shared__ R sth[3][3][QT][64];
+  __shared__ float sphi[4][QR + 1][QC];  // row QR absorbs the padding transfers
+  __shared__ R sth[3][3][QT][TC];
 
   const DevParams& dp = a.dp;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
   const size_t nxny = (size_t)nx * ny;
   const int sx = dp.sx;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int ntx = (nx + QX - 1) / QX, nty = (ny + QY - 1) / QY;
-  const int x0 = (tile % ntx) * QX, y0 = ((tile / ntx) % nty) * QY;
+  const bool edge_wave = wave == QW;
+  const int ntx = (nx + 63) / 64, nty = (ny + QY - 1) / QY;
+  const int x0 = (tile % ntx) * 64, y0 = ((tile / ntx) % nty) * QY;
   const int z0 = (tile / (ntx * nty)) * tz, zend = min(z0 + tz, nz);
-  const int xl = x0 - 2 + lane;
-  const int xw = ((xl % nx) + nx) % nx;
-  const bool out_lane = lane >= 2 && lane < 62 && xl < nx;
+  const int XW = min(64, nx - x0);
   const uint32_t pb = (uint32_t)(nxny * sizeof(TA));
   const TA* phi = (const TA*)a.phiS;
   TA* pre = (TA*)a.pre;
   const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
   const R THREE = 3, ONE = 1, TWO = 2;
   const R eps = (R)1.0e-9;
-  const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
+  auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
 
-  // this wave streams phi rows wave + QW*i; rows past the tile go to the padding row
-  uint32_t rowoff[QDMA], rowlds[QDMA];
+  // theta column of this lane: row waves j = lane + 1 (x = x0 + lane); the
+  // edge wave's lane 2t + s computes theta row t at j = 0 (s = 0) or 65
+  int tj = lane + 1, trow0 = 0;
+  bool edge_active = true;
+  if (edge_wave) {
+    edge_active = lane < 2 * QT;
+    tj = (lane & 1) ? TC - 1 : 0;
+    trow0 = edge_active ? (lane >> 1) : 0;
+  }
+  const int xt = x0 - 1 + tj;            // x of this lane's theta column (unwrapped)
+  const int xo = x0 + lane;              // output column of a row-wave lane
+  const bool out_lane = !edge_wave && lane < XW;
+
+  // DMA: row wave w streams phi rows w + QW*i: its 64 lanes to columns
+  // 2..65, lanes 0-1 to columns 0-1 (x0-2, x0-1) and to 66-67 (x0+64,
+  // x0+65); rows past the tile go to the padding row
+  // row byte offsets are wave-uniform (scalar); the lane's column offsets
+  // are added when a transfer is issued
+  uint32_t rowbyte[QDMA], rowlds[QDMA];
+  const uint32_t xm = (uint32_t)wrapx(x0 + lane) * 4u;
+  const uint32_t xlh = (uint32_t)wrapx(x0 - 2 + (lane & 1)) * 4u, xrh = (uint32_t)wrapx(x0 + 64 + (lane & 1)) * 4u;
 #pragma unroll
   for (int i = 0; i < QDMA; ++i) {
     const int r = wave + QW * i;
     const int yw = (((y0 - 2 + (r < QR ? r : 0)) % ny) + ny) % ny;
-    rowoff[i] = ((uint32_t)yw * (uint32_t)nx + (uint32_t)xw) * (uint32_t)sizeof(TA);
-    rowlds[i] = (uint32_t)(r < QR ? r : QR) * 64u * 4u;
+    rowbyte[i] = __builtin_amdgcn_readfirstlane((uint32_t)yw * (uint32_t)nx * (uint32_t)sizeof(TA));
+    rowlds[i] = __builtin_amdgcn_readfirstlane((uint32_t)(r < QR ? r : QR) * (uint32_t)QC * 4u);
   }
   const uint32_t lds_phi = lds_addr(&sphi[0][0][0]);
   auto dma_plane = [&](int z) {
+    if (edge_wave) return;
     const int zw = ((z % nz) + nz) % nz;
     const Rsrc rs = mkrsrc(phi + (size_t)zw * nxny, pb);
-    const uint32_t base = lds_phi + (uint32_t)(z & 3) * (QR + 1) * 64u * 4u;
+    const uint32_t base = lds_phi + (uint32_t)(z & 3) * (QR + 1) * QC * 4u;
 #pragma unroll
-    for (int i = 0; i < QDMA; ++i) dma_dword(rs, rowoff[i], base + rowlds[i]);
+    for (int i = 0; i < QDMA; ++i) dma_dword<8u>(rs, rowbyte[i] + xm, base + rowlds[i]);
+    // the halo columns: two lanes each (the transfers of the other lanes are
+    // masked off); three transfers per row in all
+    if (lane < 2) {
+#pragma unroll
+      for (int i = 0; i < QDMA; ++i) {
+        dma_dword<0u>(rs, rowbyte[i] + xlh, base + rowlds[i]);
+        dma_dword<66u * 4u>(rs, rowbyte[i] + xrh, base + rowlds[i]);
+      }
+    }
   };
   auto fluid = [&](int x, int y, int z) -> bool {
     if (!dp.obst) return true;
@@ -338,9 +373,9 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   dma_plane(z0);
   dma_plane(z0 + 1);
   for (int k = z0 - 1; k <= zend; ++k) {
-    // phi(k+1) has landed once only the newest plane's QDMA transfers are outstanding
+    // phi(k+1) has landed once only the newest plane's 3*QDMA transfers are outstanding
     if (k + 2 <= zend + 1)
-      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(QDMA) : "memory");
+      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * QDMA) : "memory");
     else
       dma_wait_all();
     __syncthreads();
@@ -361,7 +396,9 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
       st_d[h] = false;
     }
 
-    // ---- grad / lap / theta of plane k on theta rows wave, wave + QW -------------
+    // ---- grad / lap / theta of plane k ----------------------------------------------
+    // row waves: theta rows wave, wave + QW (columns 1..64); edge wave: rows
+    // lane >> 1 at columns 0 and 65
     static_assert(QT == 2 * QW, "two theta rows per wave");
     {
       const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
@@ -369,15 +406,14 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
       const bool own = k >= z0 && k < zend;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int t = wave + QW * h;
-        // lanes 0 and 63 read their own column for the missing neighbour;
-        // no output depends on them
-#define F(a_, b_, c_) ((R)sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][t + 1 + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)])
+        if (edge_wave && h == 1) break;  // wave-uniform: the edge wave has one row per lane
+        const int t = edge_wave ? trow0 : wave + QW * h;
+#define F(a_, b_, c_) ((R)sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][t + 1 + (b_)][tj + 1 + (a_)])
         const R gx = GRAD_X(F), gy = GRAD_Y(F), gz = GRAD_Z(F);
         const R c = F(0, 0, 0);
         const int y = y0 - 1 + t;
         R thx = 0, thy = 0, thz = 0;
-        if (fluid(xl, y, k)) {
+        if (fluid(xt, y, k)) {
           const R mg = sqrt(gx * gx + gy * gy + gz * gz);
           const R im = rcp(mg + eps);
           const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
@@ -389,27 +425,29 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
             hg[h][1] = (R)(TA)nyv * mgs;
             hg[h][2] = (R)(TA)nzv * mgs;
             hg[h][3] = LAPL(F, c);
-            og[h] = sbase(5, xl, y, k, sx, ny);
+            og[h] = sbase(5, xo, y, k, sx, ny);
             st_g[h] = true;
           }
         }
 #undef F
-        sth[tk][0][t][lane] = (R)(TA)thx;
-        sth[tk][1][t][lane] = (R)(TA)thy;
-        sth[tk][2][t][lane] = (R)(TA)thz;
+        if (edge_active) {
+          sth[tk][0][t][tj] = (R)(TA)thx;
+          sth[tk][1][t][tj] = (R)(TA)thy;
+          sth[tk][2][t][tj] = (R)(TA)thz;
+        }
       }
     }
     __syncthreads();
 
     // ---- div theta of plane k-1 on theta rows 1 + wave, 1 + wave + QW ------------
     const int zo = k - 1;
-    if (zo >= z0 && zo < zend) {
+    if (!edge_wave && zo >= z0 && zo < zend) {
       const int tm = (k + 1) % 3, t0 = (k + 2) % 3, tp = (k + 3) % 3;  // planes k-2, k-1, k
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int t = 1 + wave + QW * h;
         if (t > QY) continue;
-#define TH(f_, a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][f_][t + (b_)][(a_) < 0 ? lm : ((a_) > 0 ? lp : lane)]
+#define TH(f_, a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][f_][t + (b_)][tj + (a_)]
 #define X(a_, b_, c_) TH(0, a_, b_, c_)
 #define Y(a_, b_, c_) TH(1, a_, b_, c_)
 #define Z(a_, b_, c_) TH(2, a_, b_, c_)
@@ -419,9 +457,9 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 #undef Z
 #undef TH
         const int y = y0 - 1 + t;
-        if (out_lane && y < ny && fluid(xl, y, zo)) {
+        if (out_lane && y < ny && fluid(xo, y, zo)) {
           hd[h] = d;
-          od[h] = sbase(5, xl, y, zo, sx, ny);
+          od[h] = sbase(5, xo, y, zo, sx, ny);
           st_d[h] = true;
         }
       }
@@ -442,13 +480,13 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(QW * 64) k_phase_prep_z(StageArgs a, int tz) {
+__global__ void __launch_bounds__(QNT) k_phase_prep_z(StageArgs a, int tz) {
   prep_body_z<PREC>(a, tz, xcd_tile(blockIdx.x, prep_tiles_z(a.dp, tz)));
 }
 
 // many domains per launch (see fused_kernel_batch)
 template <int PREC>
-__global__ void __launch_bounds__(QW * 64)
+__global__ void __launch_bounds__(QNT)
     k_phase_prep_z_batch(const PrepBatchEntry* __restrict__ tab, int ndom, int total) {
   const int g = xcd_tile(blockIdx.x, total);
   int lo = 0, hi = ndom - 1;
@@ -855,7 +893,7 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
 template <int PREC>
 static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
   if constexpr (sizeof(typename Aux<PREC>::T) == 4) {
-    hipLaunchKernelGGL(k_phase_prep_z<PREC>, dim3(prep_tiles_z(a.dp, tz)), dim3(QW * 64), 0, s, a, tz);
+    hipLaunchKernelGGL(k_phase_prep_z<PREC>, dim3(prep_tiles_z(a.dp, tz)), dim3(QNT), 0, s, a, tz);
   } else {
     const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
     hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
@@ -869,7 +907,7 @@ int prep_tile_count(const StageArgs& a, int prec, int tz) {
 }
 
 hipError_t launch_phase_prep_batch(const PrepBatchEntry* dev_tab, int ndom, int total_tiles, int prec, hipStream_t s) {
-  const dim3 grid(total_tiles), block(QW * 64);
+  const dim3 grid(total_tiles), block(QNT);
   switch (prec) {
     case PREC_FP32: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_FP32>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
     case PREC_MIXED: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_MIXED>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
