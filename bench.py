@@ -3,7 +3,7 @@
 
 Workload (the metric's configuration): two-component D3Q27 HIGHORDER
 DENSRATIO, 512^3 periodic, a Laplace droplet (R = 128, w = 3, density ratio
-100, sigma = 0.01), FP32 moments in SoA layout.  A "step" is one full time
+100, sigma = 0.01), FP32 moments in row-interleaved field sets.  A "step" is one full time
 step of lbf_step (phase-field stencils + the fused stream/collide/moments/
 phase-update kernel) over the whole domain; state stays resident in HBM.
 
@@ -14,6 +14,8 @@ Prints ONE JSON line:
                   over its average launch time from HIP events on the context stream
   fp16s / bf16s   the same workload with moments stored as scaled 16-bit
                   deviations, FP32 arithmetic; speed-up and velocity error vs FP64
+  fortran_host    the same workload driven by the Fortran host (namelist deck,
+                  its own initial condition and time loop) through the C-ABI
   cpu_baseline    the oracle (a CPU port of the reference algorithm) timed
                   on this host on a bounded sample of the same workload
 """
@@ -97,9 +99,38 @@ def precision_errors(n=(96, 96, 96), steps=400):
             "vel_rel_l2": float(np.linalg.norm(u - ru) / np.linalg.norm(ru)),
             "vel_linf": float(np.abs(u - ru).max()),
             "P_rel_l2": float(np.linalg.norm(P - rP) / np.linalg.norm(rP)),
+            "P_linf": float(np.abs(P - rP).max()),
             "phi_rel_l2": float(np.linalg.norm(phi - rphi) / np.linalg.norm(rphi)),
+            "phi_linf": float(np.abs(phi - rphi).max()),
         }
     return {"grid": list(n), "steps": steps, "vs": "fp64 GPU run", "fields": out}
+
+
+def fortran_host(n, steps=100):
+    """The 512^3 droplet driven by the Fortran host (lbfast_amd/fortran): a
+    namelist deck with the bench's parameters, the host's own droplet start
+    (init_conditions_module.f90:394-426) and time loop; its printed MLUPS
+    covers the time loop only, as LBFAST.f90:495-505."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "lbfast_amd", "fortran", "bin", "lbfast_host_laplace27h_fp32")
+    if not os.path.exists(exe):
+        return {"skipped": "Fortran host not built (amdflang absent at build time)"}
+    c = n[0] / 2.0 + 0.5
+    deck = (f"&simulation\n nsteps={steps}\n lx={n[0]}\n ly={n[1]}\n lz={n[2]}\n lprint=.false.\n"
+            f" lraw=.false.\n iprobe=1\n jprobe=1\n kprobe=1\n&end\n"
+            f"&fluid\n visc1=0.1\n visc2=0.1\n rho_r=1.0\n rho_b=0.01\n tau_diff=0.1\n sigma=0.01\n"
+            f" width=3.0\n radius={n[0] / 4.0}\n center={c}, {c}, {c}\n&end\n")
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "input.inp"), "w") as f:
+            f.write(deck)
+        r = subprocess.run([exe, "input.inp"], cwd=wd, capture_output=True, text=True, timeout=600)
+    line = [l for l in r.stdout.splitlines() if "MLUPS" in l]
+    if r.returncode != 0 or not line:
+        return {"error": (r.stdout + r.stderr)[-500:]}
+    tok = line[-1].split()
+    return {"value": float(tok[tok.index("MLUPS") + 1]), "unit": "MLUPS", "steps": int(tok[tok.index("steps") + 1]),
+            "binary": os.path.basename(exe)}
 
 
 def cpu_baseline(target_s=15.0):
@@ -286,6 +317,9 @@ def main():
             ens[name], nodes_e = ensemble(prec)
         out["ensemble"] = {"domains": 96, "nodes": nodes_e, "steps": 200,
                            "workload": "96 two-component D3Q27h droplets, L in {32,48,64,96,128}", **ens}
+    if not args.only:
+        print("[bench] Fortran host", file=sys.stderr, flush=True)
+        out["fortran_host"] = fortran_host(n)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -58,8 +58,10 @@ extern "C" {
 #define LBF_PREC_FP64     0  /* PRC 8, STRPRC 8                              */
 #define LBF_PREC_FP32     1  /* PRC 4, STRPRC 4                              */
 #define LBF_PREC_MIXED    2  /* PRC 8 compute, STRPRC 4 storage ("sd")       */
-#define LBF_PREC_FP16S    3  /* FP32 compute, moments stored as scaled FP16
-                                deviations (P, u, Pi - u u); phi, u_old FP32 */
+#define LBF_PREC_FP16S    3  /* FP32 compute, moments and u_old stored as
+                                scaled FP16 deviations (P, u, Pi - u u), the
+                                time step rounding stochastically; phi and the
+                                phase-field intermediates FP32 */
 #define LBF_PREC_BF16S    4  /* as FP16S with bfloat16 storage               */
 
 /* ---- field ids ----------------------------------------------------------
