@@ -261,18 +261,31 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 // ---------------------------------------------------------------------------
 // Streaming fast-path phase-field prep (float phase-field storage).
 //
-// Same stencils and expression order as k_norm / k_div
-// (lb_cuda_auxfields_module.f90:72-130, 1110-1160), evaluated from LDS.  A
-// block owns a 64-column tile (outputs x0 .. x0+63, aligned with the field
+// The stencils of k_norm / k_div (lb_cuda_auxfields_module.f90:72-130,
+// 1110-1160) factor: the D3Q27 weights are w(a,b,c) = u_a u_b u_c / 216 with
+// u = (1, 4, 1) over offsets (-1, 0, 1) (p0 = 64/216, p1 = 16/216, p2 = 4/216,
+// p3 = 1/216), so
+//   DX(F)  = sum_c u_c sum_b u_b (F(1,b,c) - F(-1,b,c)) / 216   (DY, DZ alike)
+//   LAPL   = 6 (sum_abc w(a,b,c) F(a,b,c) - F(0,0,0)).
+// Each plane therefore contributes three 9-point partial sums per node (an
+// x difference smoothed in y, a y difference smoothed in x, and the 2-D
+// smoothing), and the z direction combines the partials of three planes,
+// which every thread keeps in registers for its own nodes as it marches z.
+// That reads 9 phi values per node from LDS (27 direct) and 21 theta values
+// (54 direct).  The sums run in a different order than the direct stencils,
+// so results agree with them to rounding, not bit for bit.
+//
+// A block owns a 64-column tile (outputs x0 .. x0+63, aligned with the field
 // rows, so every store is a whole-row segment) by QY rows and marches z.
 // Its QW row waves hold two theta rows each; one more wave computes theta on
 // the two edge columns x0-1 and x0+64 that the tile's divergence needs.
 // phi rows are kept in LDS with a 2-column halo each side (column c holds
 // x0-2+c, wrapped): the row's 64 lanes and two 2-lane transfers for the
-// halo columns stream in by LDS DMA three planes ahead, into a ring of four
-// planes; theta lives in a ring of three planes of 66 columns (x0-1 ..
-// x0+64).  Iteration k computes grad/lap/theta of plane k and div theta of
-// plane k-1.  isfluid is read only with internal obstacles.
+// halo columns stream in by LDS DMA two planes ahead, into a ring of four
+// planes; theta lives in a ring of two planes of 66 columns (x0-1 ..
+// x0+64).  Iteration k reads phi plane k+1 and computes grad/lap/theta of
+// plane k and div theta of plane k-1.  isfluid is read only with internal
+// obstacles.
 constexpr int QY = 12, QW = 7;
 constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
 constexpr int QT = QY + 2;                 // theta rows (1-row halo)
@@ -285,13 +298,32 @@ __host__ __device__ inline int prep_tiles_z(const DevParams& dp, int tz) {
   return ((dp.nx + 63) / 64) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
 }
 
+// the three partial sums of one plane at one node (see above)
+template <class R> struct Part3 {
+  R dx, dy, s;  // x difference smoothed in y, y difference smoothed in x, 2-D smoothing
+};
+
+// partials of the 3 x 3 neighbourhood v[b][a] (b: y offset + 1, a: x offset + 1)
+template <class R>
+__device__ __forceinline__ Part3<R> part3(const R (&v)[3][3]) {
+  const R FOUR = 4;
+  Part3<R> q;
+  q.dx = (v[0][2] - v[0][0]) + FOUR * (v[1][2] - v[1][0]) + (v[2][2] - v[2][0]);
+  q.dy = (v[2][0] - v[0][0]) + FOUR * (v[2][1] - v[0][1]) + (v[2][2] - v[0][2]);
+  const R r0 = v[0][0] + FOUR * v[0][1] + v[0][2];
+  const R r1 = v[1][0] + FOUR * v[1][1] + v[1][2];
+  const R r2 = v[2][0] + FOUR * v[2][1] + v[2][2];
+  q.s = r0 + FOUR * r1 + r2;
+  return q;
+}
+
 template <int PREC>
 __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile) {
   typedef typename Codec<PREC>::R R;
   typedef typename Aux<PREC>::T TA;
   static_assert(sizeof(TA) == 4, "float phase-field storage");
   __shared__ float sphi[4][QR + 1][QC];  // row QR absorbs the padding transfers
-  __shared__ R sth[3][3][QT][TC];
+  __shared__ R sth[2][3][QT][TC];
 
   const DevParams& dp = a.dp;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
@@ -306,8 +338,8 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   const uint32_t pb = (uint32_t)(nxny * sizeof(TA));
   const TA* phi = (const TA*)a.phiS;
   TA* pre = (TA*)a.pre;
-  const R p1 = (R)(2.0 / 27.0), p2 = (R)(1.0 / 54.0), p3 = (R)(1.0 / 216.0), p0 = (R)(8.0 / 27.0);
-  const R THREE = 3, ONE = 1, TWO = 2;
+  const R FOUR = 4, ONE = 1;
+  const R G3 = (R)(3.0 / 216.0), L6 = (R)(6.0 / 216.0), SIX = 6;
   const R eps = (R)1.0e-9;
   auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
 
@@ -363,6 +395,11 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
     return v == 1 || v == -1;
   };
 
+  // per theta row h of this lane: phi partials of planes k-1 and k (pm, p0)
+  // and phi at the node in plane k (c0); per div row: theta partials of
+  // planes k-2 and k-1 (em, e0)
+  Part3<R> pm[2], p0[2], em[2], e0[2];
+  R c0[2];
   // outputs held for one iteration (see below)
   R hg[2][4], hd[2];
   size_t og[2] = {0, 0}, od[2] = {0, 0};
@@ -370,9 +407,8 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 
   dma_plane(z0 - 2);
   dma_plane(z0 - 1);
-  dma_plane(z0);
-  dma_plane(z0 + 1);
-  for (int k = z0 - 1; k <= zend; ++k) {
+  // iterations z0-3 and z0-2 only fill the partials of planes z0-2, z0-1
+  for (int k = z0 - 3; k <= zend; ++k) {
     // phi(k+1) has landed once only the newest plane's 3*QDMA transfers are outstanding
     if (k + 2 <= zend + 1)
       asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * QDMA) : "memory");
@@ -396,75 +432,103 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
       st_d[h] = false;
     }
 
-    // ---- grad / lap / theta of plane k ----------------------------------------------
+    // ---- phi partials of plane k+1; grad / lap / theta of plane k -----------------
     // row waves: theta rows wave, wave + QW (columns 1..64); edge wave: rows
     // lane >> 1 at columns 0 and 65
     static_assert(QT == 2 * QW, "two theta rows per wave");
     {
-      const int sm = (k - 1) & 3, s0 = k & 3, sp = (k + 1) & 3;
-      const int tk = (k + 3) % 3;
+      const int sp = (k + 1) & 3, tk = k & 1;
+      const bool have_theta = k >= z0 - 1;
       const bool own = k >= z0 && k < zend;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (edge_wave && h == 1) break;  // wave-uniform: the edge wave has one row per lane
         const int t = edge_wave ? trow0 : wave + QW * h;
-#define F(a_, b_, c_) ((R)sphi[(c_) < 0 ? sm : ((c_) > 0 ? sp : s0)][t + 1 + (b_)][tj + 1 + (a_)])
-        const R gx = GRAD_X(F), gy = GRAD_Y(F), gz = GRAD_Z(F);
-        const R c = F(0, 0, 0);
-        const int y = y0 - 1 + t;
-        R thx = 0, thy = 0, thz = 0;
-        if (fluid(xt, y, k)) {
-          const R mg = sqrt(gx * gx + gy * gy + gz * gz);
-          const R im = rcp(mg + eps);
-          const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
-          thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
-          if (own && out_lane && t >= 1 && t <= QY && y < ny) {
-            // the reference re-reads n and |g| from storage precision
-            const R mgs = (R)(TA)mg;
-            hg[h][0] = (R)(TA)nxv * mgs;
-            hg[h][1] = (R)(TA)nyv * mgs;
-            hg[h][2] = (R)(TA)nzv * mgs;
-            hg[h][3] = LAPL(F, c);
-            og[h] = sbase(5, xo, y, k, sx, ny);
-            st_g[h] = true;
+        R v[3][3];
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) v[bb][aa] = (R)sphi[sp][t + bb][tj + aa];
+        const Part3<R> pp = part3(v);
+        const R cp = v[1][1];
+        if (have_theta) {
+          const R gx = G3 * (pm[h].dx + FOUR * p0[h].dx + pp.dx);
+          const R gy = G3 * (pm[h].dy + FOUR * p0[h].dy + pp.dy);
+          const R gz = G3 * (pp.s - pm[h].s);
+          const R c = c0[h];
+          const int y = y0 - 1 + t;
+          R thx = 0, thy = 0, thz = 0;
+          if (fluid(xt, y, k)) {
+            const R mg = sqrt(gx * gx + gy * gy + gz * gz);
+            const R im = rcp(mg + eps);
+            const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
+            thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
+            if (own && out_lane && t >= 1 && t <= QY && y < ny) {
+              // the reference re-reads n and |g| from storage precision
+              const R mgs = (R)(TA)mg;
+              hg[h][0] = (R)(TA)nxv * mgs;
+              hg[h][1] = (R)(TA)nyv * mgs;
+              hg[h][2] = (R)(TA)nzv * mgs;
+              hg[h][3] = L6 * (pm[h].s + FOUR * p0[h].s + pp.s) - SIX * c;
+              og[h] = sbase(5, xo, y, k, sx, ny);
+              st_g[h] = true;
+            }
+          }
+          if (edge_active) {
+            sth[tk][0][t][tj] = (R)(TA)thx;
+            sth[tk][1][t][tj] = (R)(TA)thy;
+            sth[tk][2][t][tj] = (R)(TA)thz;
           }
         }
-#undef F
-        if (edge_active) {
-          sth[tk][0][t][tj] = (R)(TA)thx;
-          sth[tk][1][t][tj] = (R)(TA)thy;
-          sth[tk][2][t][tj] = (R)(TA)thz;
-        }
+        pm[h] = p0[h];
+        p0[h] = pp;
+        c0[h] = cp;
       }
     }
     __syncthreads();
 
-    // ---- div theta of plane k-1 on theta rows 1 + wave, 1 + wave + QW ------------
+    // ---- theta partials of plane k; div theta of plane k-1 on theta rows 1 + wave,
+    // 1 + wave + QW ------------------------------------------------------------------
     const int zo = k - 1;
-    if (!edge_wave && zo >= z0 && zo < zend) {
-      const int tm = (k + 1) % 3, t0 = (k + 2) % 3, tp = (k + 3) % 3;  // planes k-2, k-1, k
+    if (!edge_wave && k >= z0 - 1) {
+      const int tk = k & 1;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int t = 1 + wave + QW * h;
         if (t > QY) continue;
-#define TH(f_, a_, b_, c_) sth[(c_) < 0 ? tm : ((c_) > 0 ? tp : t0)][f_][t + (b_)][tj + (a_)]
-#define X(a_, b_, c_) TH(0, a_, b_, c_)
-#define Y(a_, b_, c_) TH(1, a_, b_, c_)
-#define Z(a_, b_, c_) TH(2, a_, b_, c_)
-        const R d = (DX(X) + DY(Y) + DZ(Z)) * THREE;
-#undef X
-#undef Y
-#undef Z
-#undef TH
-        const int y = y0 - 1 + t;
-        if (out_lane && y < ny && fluid(xo, y, zo)) {
-          hd[h] = d;
-          od[h] = sbase(5, xo, y, zo, sx, ny);
-          st_d[h] = true;
+        R w[3][3];
+        Part3<R> ep;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][0][t - 1 + bb][tj - 1 + aa];
+        ep.dx = (w[0][2] - w[0][0]) + FOUR * (w[1][2] - w[1][0]) + (w[2][2] - w[2][0]);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][1][t - 1 + bb][tj - 1 + aa];
+        ep.dy = (w[2][0] - w[0][0]) + FOUR * (w[2][1] - w[0][1]) + (w[2][2] - w[0][2]);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][2][t - 1 + bb][tj - 1 + aa];
+        ep.s = (w[0][0] + FOUR * w[0][1] + w[0][2]) + FOUR * (w[1][0] + FOUR * w[1][1] + w[1][2]) +
+               (w[2][0] + FOUR * w[2][1] + w[2][2]);
+        if (zo >= z0 && zo < zend) {
+          const R d = G3 * ((em[h].dx + FOUR * e0[h].dx + ep.dx) + (em[h].dy + FOUR * e0[h].dy + ep.dy) +
+                            (ep.s - em[h].s));
+          const int y = y0 - 1 + t;
+          if (out_lane && y < ny && fluid(xo, y, zo)) {
+            hd[h] = d;
+            od[h] = sbase(5, xo, y, zo, sx, ny);
+            st_d[h] = true;
+          }
         }
+        em[h] = e0[h];
+        e0[h] = ep;
       }
     }
-    // slot (k+3)&3 held plane k-1, read above
+    // slot (k+3)&3 held plane k-1, last read in iteration k-2
     if (k + 3 <= zend + 1) dma_plane(k + 3);
   }
 #pragma unroll
