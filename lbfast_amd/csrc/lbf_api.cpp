@@ -523,6 +523,7 @@ static StageArgs stage_args(lbf_ctx* c) {
   a.frc = c->frc;
   a.pre = c->pre;
   a.gf_from_pre = 0;
+  a.div_only = fused_grad_inline(c->prec, c->dp) ? 1 : 0;
   a.rep = c->rep;
   a.rep_mask = c->rep_mask;
   a.isfh = c->isfh;

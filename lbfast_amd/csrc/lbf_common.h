@@ -145,6 +145,46 @@ __device__ __forceinline__ void dma_dword(Rsrc r, uint32_t voff, uint32_t lds) {
 __device__ __forceinline__ void dma_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void lds_wait_all() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// Separable D3Q27 stencils.  The isotropic weights of the gradient and
+// Laplacian stencils (lb_cuda_auxfields_module.f90:72-130) are
+// w(a,b,c) = u_a u_b u_c / 216 with u = (1, 4, 1) over offsets (-1, 0, 1), so
+//   DX(F) = sum_c u_c sum_b u_b (F(1,b,c) - F(-1,b,c)) / 216   (DY, DZ alike)
+//   LAPL  = 6 (sum_abc w(a,b,c) F(a,b,c) - F(0,0,0)).
+// One z plane contributes three 9-point partial sums per node; the z
+// direction combines the partials of three consecutive planes:
+//   grad = 3/216 (dx_m + 4 dx_0 + dx_p,  dy_m + 4 dy_0 + dy_p,  s_p - s_m)
+//   lap  = 6/216 (s_m + 4 s_0 + s_p) - 6 F(0,0,0)
+// the three partial sums of one plane at one node
+template <class R> struct Part3 {
+  R dx, dy, s;  // x difference smoothed in y, y difference smoothed in x, 2-D smoothing
+};
+
+// partials of the 3 x 3 neighbourhood v[b][a] (b: y offset + 1, a: x offset + 1)
+template <class R>
+__device__ __forceinline__ Part3<R> part3(const R (&v)[3][3]) {
+  const R FOUR = 4;
+  Part3<R> q;
+  q.dx = (v[0][2] - v[0][0]) + FOUR * (v[1][2] - v[1][0]) + (v[2][2] - v[2][0]);
+  q.dy = (v[2][0] - v[0][0]) + FOUR * (v[2][1] - v[0][1]) + (v[2][2] - v[0][2]);
+  const R r0 = v[0][0] + FOUR * v[0][1] + v[0][2];
+  const R r1 = v[1][0] + FOUR * v[1][1] + v[1][2];
+  const R r2 = v[2][0] + FOUR * v[2][1] + v[2][2];
+  q.s = r0 + FOUR * r1 + r2;
+  return q;
+}
+
+// gradient and Laplacian at a node from the partials of planes z-1, z, z+1
+// (m, o, p) and the node's own value c
+template <class R>
+__device__ __forceinline__ void grad_lap(const Part3<R>& m, const Part3<R>& o, const Part3<R>& p, R c, R& gx, R& gy,
+                                         R& gz, R& lap) {
+  const R FOUR = 4, SIX = 6, G3 = (R)(3.0 / 216.0), L6 = (R)(6.0 / 216.0);
+  gx = G3 * (m.dx + FOUR * o.dx + p.dx);
+  gy = G3 * (m.dy + FOUR * o.dy + p.dy);
+  gz = G3 * (p.s - m.s);
+  lap = L6 * (m.s + FOUR * o.s + p.s) - SIX * c;
+}
+
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs (workgroup b runs on XCD b % 8), each with its own L2.  Give XCD k a
 // contiguous range of tiles in (x fastest, y, z) order, so the tiles resident

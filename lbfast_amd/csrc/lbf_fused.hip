@@ -168,11 +168,17 @@ struct RawNode {
 };
 
 // LDS slots of the prefetched inputs (one 64-lane row of dwords each):
-// moments 0-9, phi 10, g 11-15, u_old 16-18.
-template <bool TWO, bool INLINE>
+// moments 0-9, phi 10, g 11-15, u_old 16-18.  GI (gradient inline): phi
+// comes from the block's phi tile instead, and g holds div theta only (slot
+// 10, u_old 11-13).
+template <bool TWO, bool INLINE, bool GI = false>
 struct Slots {
   static constexpr int PHI = 10, G = 11, UO = 16;
   static constexpr int N = INLINE ? (TWO ? 19 : 10) : 14;
+};
+template <>
+struct Slots<true, true, true> {
+  static constexpr int PHI = -1, G = 10, UO = 11, N = 14;
 };
 
 // Per-lane byte offsets of one node inside a z plane of each input buffer,
@@ -219,9 +225,9 @@ __device__ __forceinline__ void load_raw(RawNode<PREC>& r, const FusedArgs& a, i
 // dword that holds the lane's element (the half is picked when read back:
 // byte (o.m & 2) of the slot; sx is even, so every 16-bit field row starts on
 // a dword and the u_old offsets share that parity).
-template <int PREC, bool TWO, bool INLINE>
+template <int PREC, bool TWO, bool INLINE, bool GI>
 __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, const FOff& o, uint32_t lds0) {
-  typedef Slots<TWO, INLINE> SL;
+  typedef Slots<TWO, INLINE, GI> SL;
   typedef typename Codec<PREC>::T T;
   static_assert(sizeof(typename Aux<PREC>::T) == 4, "float phase-field storage");
   constexpr int GF = INLINE ? 5 : 3;
@@ -240,7 +246,7 @@ __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, c
   dma_dword<256u * 7>(mkrsrc(S + 7 * fs, o.pm), vM, lds0);
   dma_dword<256u * 8>(mkrsrc(S + 8 * fs, o.pm), vM, lds0);
   dma_dword<256u * 9>(mkrsrc(S + 9 * fs, o.pm), vM, lds0);
-  if constexpr (TWO) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + rows * dp.nx, o.pp), o.p, lds0);
+  if constexpr (TWO && !GI) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + rows * dp.nx, o.pp), o.p, lds0);
   if constexpr (!INLINE) {
     const float* fr = (const float*)a.frc + rows * GF * fs;
     dma_dword<256u * (SL::G + 0)>(mkrsrc(fr + 0 * fs, o.pg), o.g, lds0);
@@ -248,11 +254,15 @@ __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, c
     dma_dword<256u * (SL::G + 2)>(mkrsrc(fr + 2 * fs, o.pg), o.g, lds0);
   } else if constexpr (TWO) {
     const float* pr = (const float*)a.pre + rows * GF * fs;
-    dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * fs, o.pg), o.g, lds0);
-    dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * fs, o.pg), o.g, lds0);
-    dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * fs, o.pg), o.g, lds0);
-    dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * fs, o.pg), o.g, lds0);
-    dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * fs, o.pg), o.g, lds0);
+    if constexpr (GI) {
+      dma_dword<256u * SL::G>(mkrsrc(pr + 4 * fs, o.pg), o.g, lds0);
+    } else {
+      dma_dword<256u * (SL::G + 0)>(mkrsrc(pr + 0 * fs, o.pg), o.g, lds0);
+      dma_dword<256u * (SL::G + 1)>(mkrsrc(pr + 1 * fs, o.pg), o.g, lds0);
+      dma_dword<256u * (SL::G + 2)>(mkrsrc(pr + 2 * fs, o.pg), o.g, lds0);
+      dma_dword<256u * (SL::G + 3)>(mkrsrc(pr + 3 * fs, o.pg), o.g, lds0);
+      dma_dword<256u * (SL::G + 4)>(mkrsrc(pr + 4 * fs, o.pg), o.g, lds0);
+    }
     if (dp.dens) {
       const T* uo = (const T*)a.uin + rows * 3 * fs;
       dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * fs, o.pu), vU, lds0);
@@ -262,19 +272,25 @@ __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, c
   }
 }
 
-template <int PREC, bool TWO, bool INLINE>
+template <int PREC, bool TWO, bool INLINE, bool GI>
 __device__ __forceinline__ void lds_raw(RawNode<PREC>& r, const float (*slot)[64], int lane, bool dens, uint32_t bT) {
-  typedef Slots<TWO, INLINE> SL;
+  typedef Slots<TWO, INLINE, GI> SL;
   typedef typename Codec<PREC>::T T;
   // a 16-bit element sits at byte (bT & 2) of its lane's dword
   const uint32_t hs = sizeof(T) == 2 ? (bT & 2u) : 0u;
   auto mom = [&](int f) { return *(const T*)((const char*)&slot[f][lane] + hs); };
 #pragma unroll
   for (int f = 0; f < 10; ++f) r.m[f] = mom(f);
-  if constexpr (TWO) r.phi = slot[SL::PHI][lane];
+  if constexpr (TWO && !GI) r.phi = slot[SL::PHI][lane];
   if constexpr (!INLINE) {
 #pragma unroll
     for (int f = 0; f < 3; ++f) r.g[f] = slot[SL::G + f][lane];
+  } else if constexpr (GI) {
+    r.g[4] = slot[SL::G][lane];
+    if (dens) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) r.uo[f] = mom(SL::UO + f);
+    }
   } else if constexpr (TWO) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) r.g[f] = slot[SL::G + f][lane];
@@ -308,12 +324,19 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef typename Aux<PREC>::T TA;
   constexpr int NW = TY + 2;  // row waves (TY output rows + 2 halo rows)
   constexpr bool PF = PREC == PREC_FP32 || PREC == PREC_FP16S || PREC == PREC_BF16S;
-  typedef Slots<TWO, INLINE> SL;
+  // GI: grad phi and lap phi formed here from a phi tile (fused_grad_inline)
+  constexpr bool GI = PF && TWO && INLINE && !OB;
+  typedef Slots<TWO, INLINE, GI> SL;
+  // GI phi tile: rows y0-2 .. y0+TY+1 (tile row r holds y0-2+r), columns
+  // x0-2 .. x0+65 (column c holds x0-2+c), both wrapped; a ring of four z
+  // planes streamed by LDS DMA three planes ahead of the plane collided
+  constexpr int GR = TY + 4, GC = 68;
 
   __shared__ R sUP[NW][9][64];
   __shared__ R sDN[NW][9][64];
   __shared__ R sHX[NW][2][9];
   __shared__ float sPF[PF ? NW + 1 : 1][PF ? SL::N : 1][64];
+  __shared__ float sPH[GI ? 4 : 1][GI ? GR : 1][GI ? GC : 1];
 
   const DevParams& dp = a.dp;
   const Phys<R>& ph = dp.ph<R>();
@@ -390,8 +413,71 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   int isf_next = 1;
   if constexpr (PF) {
     lds0 = lds_addr(&sPF[wave][0][0]);
-    dma_raw<PREC, TWO, INLINE>(a, wrapg(z0 - 1, nz), (size_t)sx, off, lds0);
+    dma_raw<PREC, TWO, INLINE, GI>(a, wrapg(z0 - 1, nz), (size_t)sx, off, lds0);
     if (obst) isf_next = isf_load(z0 - 1);
+  }
+
+  // ---- GI: phi tile stream and the per-node partial sums ------------------------
+  // Wave v streams tile row v (v = 0 .. TY+2; the x-halo wave is v = NW), wave
+  // 0 also row TY+3: 64 lanes to columns 2..65 and lanes 0-1 to columns 0-1
+  // and 66-67.  Plane p lands in slot p & 3.  Iteration zsrc reads plane
+  // zsrc+1 (issued in iteration zsrc-2, waited for at the top of zsrc-1 and
+  // published by that iteration's barriers) and streams plane zsrc+3 into the
+  // slot of plane zsrc-1, which every wave finished reading in iteration
+  // zsrc-2.
+  uint32_t ph_row1 = 0, ph_row2 = 0, ph_xm = 0, ph_xl = 0, ph_xr = 0;
+  const uint32_t lds_ph = GI ? lds_addr(&sPH[0][0][0]) : 0u;
+  auto modn = [](int i, int n) { return ((i % n) + n) % n; };
+  if constexpr (GI) {
+    ph_row1 = __builtin_amdgcn_readfirstlane((uint32_t)modn(y0 - 2 + wave, ny) * (uint32_t)nx * 4u);
+    ph_row2 = __builtin_amdgcn_readfirstlane((uint32_t)modn(y0 - 2 + GR - 1, ny) * (uint32_t)nx * 4u);
+    ph_xm = (uint32_t)modn(x0 + lane, nx) * 4u;
+    ph_xl = (uint32_t)modn(x0 - 2 + (lane & 1), nx) * 4u;
+    ph_xr = (uint32_t)modn(x0 + 64 + (lane & 1), nx) * 4u;
+  }
+  auto dma_phi = [&](int z) {
+    const Rsrc rs = mkrsrc((const float*)a.phin + (size_t)modn(z, nz) * nxny, (uint32_t)(nxny * 4u));
+    const uint32_t base = lds_ph + (uint32_t)(z & 3) * (uint32_t)(GR * GC * 4);
+    const uint32_t r1 = base + (uint32_t)wave * (uint32_t)(GC * 4);
+    dma_dword<8u>(rs, ph_row1 + ph_xm, r1);
+    if (lane < 2) {
+      dma_dword<0u>(rs, ph_row1 + ph_xl, r1);
+      dma_dword<66u * 4u>(rs, ph_row1 + ph_xr, r1);
+    }
+    if (wave == 0) {
+      const uint32_t r2 = base + (uint32_t)(GR - 1) * (uint32_t)(GC * 4);
+      dma_dword<8u>(rs, ph_row2 + ph_xm, r2);
+      if (lane < 2) {
+        dma_dword<0u>(rs, ph_row2 + ph_xl, r2);
+        dma_dword<66u * 4u>(rs, ph_row2 + ph_xr, r2);
+      }
+    }
+  };
+  // this lane's node sits at tile row w + 1, column pc
+  const int pc = halo_wave ? (side ? XW + 2 : 1) : lane + 2;
+  auto phi_part = [&](int z, R& ctr) -> Part3<R> {
+    const int sl = z & 3;
+    R v[3][3];
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa) v[bb][aa] = (R)sPH[sl][w + bb][pc - 1 + aa];
+    ctr = v[1][1];
+    return part3(v);
+  };
+  Part3<R> gpm{}, gp0{};  // partials of planes zsrc-1 and zsrc
+  R gc0 = 0;              // phi at the node in plane zsrc
+  if constexpr (GI) {
+    dma_phi(z0 - 2);
+    dma_phi(z0 - 1);
+    dma_phi(z0);
+    dma_phi(z0 + 1);
+    dma_wait_all();
+    __syncthreads();
+    R c_unused;
+    gpm = phi_part(z0 - 2, c_unused);
+    gp0 = phi_part(z0 - 1, gc0);
+    __syncthreads();  // the slots of planes z0-2 and z0-1 are refilled in the first iterations
   }
 
   for (int zsrc = z0 - 1; zsrc <= zend; ++zsrc) {
@@ -414,7 +500,25 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2, fs);
 #pragma unroll
       for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
-      lds_raw<PREC, TWO, INLINE>(cur, sPF[wave], lane, dp.dens, off.m);
+      lds_raw<PREC, TWO, INLINE, GI>(cur, sPF[wave], lane, dp.dens, off.m);
+      if constexpr (GI) {
+        // the prep's fields 0-3 (prep_body_z, stored at storage precision)
+        R cp;
+        const Part3<R> gpp = phi_part(zsrc + 1, cp);
+        R gx, gy, gz, lap;
+        grad_lap(gpm, gp0, gpp, gc0, gx, gy, gz, lap);
+        const R mg = sqrt(gx * gx + gy * gy + gz * gz);
+        const R im = rcp(mg + (R)1.0e-9);
+        const R mgs = (R)(TA)mg;
+        cur.g[0] = (TA)((R)(TA)(gx * im) * mgs);
+        cur.g[1] = (TA)((R)(TA)(gy * im) * mgs);
+        cur.g[2] = (TA)((R)(TA)(gz * im) * mgs);
+        cur.g[3] = (TA)lap;
+        cur.phi = (TA)gc0;
+        gpm = gp0;
+        gp0 = gpp;
+        gc0 = cp;
+      }
     } else {
       if (obst) isf = (int8_t)isf_load(zsrc);
       load_raw<PREC, TWO, INLINE>(cur, a, zw, fs, off);
@@ -529,9 +633,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         // every store of this iteration is issued: stream the next plane in
         lds_wait_all();
         if (zsrc < zend) {
-          dma_raw<PREC, TWO, INLINE>(a, wrapg(zsrc + 1, nz), fs, off, lds0);
+          dma_raw<PREC, TWO, INLINE, GI>(a, wrapg(zsrc + 1, nz), fs, off, lds0);
           if (obst) isf_next = isf_load(zsrc + 1);
         }
+        if (GI && zsrc + 3 <= zend + 1) dma_phi(zsrc + 3);
       }
       collide<LAT, HO, R>(in, f);
       if (bb && isf == -1 && out_node && own_plane) {

@@ -34,11 +34,21 @@ struct StageArgs {
   void* frc;         // 3-field set
   void* pre;         // 5-field set of the fast path (phase-field prep output)
   int gf_from_pre;   // PHI_int_boundary: donor gradient from pre instead of aux
+  int div_only;      // phase prep: store div theta only (fused_grad_inline)
   void* rep;         // REPULSIVE_FLUX: 6-field set, locaux 3..8 (partner index, flux)
   int8_t* rep_mask;  // REPULSIVE_FLUX: nx*ny*nz
   const int8_t* isfh;
   DevParams dp;
 };
+
+// The fused kernel forms grad phi and lap phi itself from phi (the prep
+// then stores only div theta) for the prefetching precisions without
+// internal obstacles: with obstacles, PHI_int_boundary rewrites phi on solid
+// nodes between the prep and the fused kernel, and the reference's gradients
+// are those of the phi before it.
+inline bool fused_grad_inline(int prec, const DevParams& dp) {
+  return (prec == PREC_FP32 || prec == PREC_FP16S || prec == PREC_BF16S) && dp.twocomp && !dp.obst;
+}
 
 // Batched launches: one grid over many domains (same lattice, precision,
 // component count and row tile; any sizes and physical parameters).

@@ -262,18 +262,13 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 // Streaming fast-path phase-field prep (float phase-field storage).
 //
 // The stencils of k_norm / k_div (lb_cuda_auxfields_module.f90:72-130,
-// 1110-1160) factor: the D3Q27 weights are w(a,b,c) = u_a u_b u_c / 216 with
-// u = (1, 4, 1) over offsets (-1, 0, 1) (p0 = 64/216, p1 = 16/216, p2 = 4/216,
-// p3 = 1/216), so
-//   DX(F)  = sum_c u_c sum_b u_b (F(1,b,c) - F(-1,b,c)) / 216   (DY, DZ alike)
-//   LAPL   = 6 (sum_abc w(a,b,c) F(a,b,c) - F(0,0,0)).
-// Each plane therefore contributes three 9-point partial sums per node (an
-// x difference smoothed in y, a y difference smoothed in x, and the 2-D
-// smoothing), and the z direction combines the partials of three planes,
-// which every thread keeps in registers for its own nodes as it marches z.
-// That reads 9 phi values per node from LDS (27 direct) and 21 theta values
-// (54 direct).  The sums run in a different order than the direct stencils,
-// so results agree with them to rounding, not bit for bit.
+// 1110-1160) in their separable form (part3 / grad_lap in lbf_common.h):
+// each plane contributes three 9-point partial sums per node, and every
+// thread keeps the partials of the last planes of its own nodes in
+// registers as it marches z.  That reads 9 phi values per node from LDS (27
+// direct) and 21 theta values (54 direct).  The sums run in a different
+// order than the direct stencils, so results agree with them to rounding,
+// not bit for bit.
 //
 // A block owns a 64-column tile (outputs x0 .. x0+63, aligned with the field
 // rows, so every store is a whole-row segment) by QY rows and marches z.
@@ -285,7 +280,8 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 // planes; theta lives in a ring of two planes of 66 columns (x0-1 ..
 // x0+64).  Iteration k reads phi plane k+1 and computes grad/lap/theta of
 // plane k and div theta of plane k-1.  isfluid is read only with internal
-// obstacles.
+// obstacles.  With a.div_only the gradient and Laplacian are not stored
+// (the fused kernel forms them itself from phi); only div theta is.
 constexpr int QY = 12, QW = 7;
 constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
 constexpr int QT = QY + 2;                 // theta rows (1-row halo)
@@ -296,25 +292,6 @@ constexpr int QNT = (QW + 1) * 64;         // threads: row waves + the edge-colu
 
 __host__ __device__ inline int prep_tiles_z(const DevParams& dp, int tz) {
   return ((dp.nx + 63) / 64) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
-}
-
-// the three partial sums of one plane at one node (see above)
-template <class R> struct Part3 {
-  R dx, dy, s;  // x difference smoothed in y, y difference smoothed in x, 2-D smoothing
-};
-
-// partials of the 3 x 3 neighbourhood v[b][a] (b: y offset + 1, a: x offset + 1)
-template <class R>
-__device__ __forceinline__ Part3<R> part3(const R (&v)[3][3]) {
-  const R FOUR = 4;
-  Part3<R> q;
-  q.dx = (v[0][2] - v[0][0]) + FOUR * (v[1][2] - v[1][0]) + (v[2][2] - v[2][0]);
-  q.dy = (v[2][0] - v[0][0]) + FOUR * (v[2][1] - v[0][1]) + (v[2][2] - v[0][2]);
-  const R r0 = v[0][0] + FOUR * v[0][1] + v[0][2];
-  const R r1 = v[1][0] + FOUR * v[1][1] + v[1][2];
-  const R r2 = v[2][0] + FOUR * v[2][1] + v[2][2];
-  q.s = r0 + FOUR * r1 + r2;
-  return q;
 }
 
 template <int PREC>
@@ -338,9 +315,9 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   const uint32_t pb = (uint32_t)(nxny * sizeof(TA));
   const TA* phi = (const TA*)a.phiS;
   TA* pre = (TA*)a.pre;
-  const R FOUR = 4, ONE = 1;
-  const R G3 = (R)(3.0 / 216.0), L6 = (R)(6.0 / 216.0), SIX = 6;
+  const R FOUR = 4, ONE = 1, G3 = (R)(3.0 / 216.0);
   const R eps = (R)1.0e-9;
+  const bool store_g = !a.div_only;
   auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
 
   // theta column of this lane: row waves j = lane + 1 (x = x0 + lane); the
@@ -452,10 +429,9 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
         const Part3<R> pp = part3(v);
         const R cp = v[1][1];
         if (have_theta) {
-          const R gx = G3 * (pm[h].dx + FOUR * p0[h].dx + pp.dx);
-          const R gy = G3 * (pm[h].dy + FOUR * p0[h].dy + pp.dy);
-          const R gz = G3 * (pp.s - pm[h].s);
           const R c = c0[h];
+          R gx, gy, gz, lap;
+          grad_lap(pm[h], p0[h], pp, c, gx, gy, gz, lap);
           const int y = y0 - 1 + t;
           R thx = 0, thy = 0, thz = 0;
           if (fluid(xt, y, k)) {
@@ -463,13 +439,13 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
             const R im = rcp(mg + eps);
             const R nxv = gx * im, nyv = gy * im, nzv = gz * im;
             thx = c * (ONE - c) * nxv; thy = c * (ONE - c) * nyv; thz = c * (ONE - c) * nzv;
-            if (own && out_lane && t >= 1 && t <= QY && y < ny) {
+            if (store_g && own && out_lane && t >= 1 && t <= QY && y < ny) {
               // the reference re-reads n and |g| from storage precision
               const R mgs = (R)(TA)mg;
               hg[h][0] = (R)(TA)nxv * mgs;
               hg[h][1] = (R)(TA)nyv * mgs;
               hg[h][2] = (R)(TA)nzv * mgs;
-              hg[h][3] = L6 * (pm[h].s + FOUR * p0[h].s + pp.s) - SIX * c;
+              hg[h][3] = lap;
               og[h] = sbase(5, xo, y, k, sx, ny);
               st_g[h] = true;
             }
