@@ -159,3 +159,21 @@ def test_tiling_edges(n):
 def test_tile_shapes(tile):
     res, bad = run_pair("drop27h", PREC_FP64, (32, 32, 32), 4, tile=tile)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("n", [(96, 20, 10), (16, 40, 12), (130, 9, 7)])
+def test_two_component_tiling_edges_fp32(n):
+    """The FP32 two-component fast path without obstacles, where the fused
+    kernel forms grad phi and lap phi from its own phi tile and the prep
+    stores div theta only: partial x tiles, rows shorter than a wave, ny
+    not a multiple of the row tile, few planes."""
+    res, bad = run_pair("drop27h", PREC_FP32, n, 5)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("tile", [(8, 1), (13, 2), (13, 5)])
+def test_two_component_tile_shapes_fp32(tile):
+    """z-chunks of one and two planes: the phi-tile ring starts and ends
+    inside the chunk's first iterations."""
+    res, bad = run_pair("drop27h", PREC_FP32, (32, 32, 32), 4, tile=tile)
+    assert not bad, bad
