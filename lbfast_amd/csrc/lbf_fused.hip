@@ -189,7 +189,20 @@ struct Slots<true, true, true> {
 struct FOff {
   uint32_t m, u, g, p;
   uint32_t pm, pu, pg, pp;
+  // 16-bit storage, full 64-node row segments (pair != 0, wave-uniform;
+  // lane l holds node x0 + l): one transfer moves two fields' rows, lanes
+  // 0-31 the dwords of field f (nodes x0 .. x0+63) and lanes 32-63 those of
+  // field f+1 (pair_delta)
+  int pair;
 };
+
+// byte offset from a lane's own 16-bit element to its dword of a paired
+// transfer: lane l < 32 fetches elements x0+2l, x0+2l+1 of field f, lane
+// l >= 32 elements x0+2(l-32), +1 of field f+1 (sx elements further)
+__device__ __forceinline__ uint32_t pair_delta(int sx) {
+  const uint32_t l = threadIdx.x & 63;
+  return 2u * (l < 32 ? l : l - 64u + (uint32_t)sx);
+}
 
 // Direct loads of one plane's inputs (used when the inputs are not prefetched).
 // fs = sx, the field stride of the sets.
@@ -236,16 +249,27 @@ __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, c
   const uint32_t vM = o.m & ~3u, vU = o.u & ~3u;
   const T* S = (const T*)a.min + rows * 10 * fs;
   // slot offsets are immediates: only the wave's LDS base stays live
-  dma_dword<256u * 0>(mkrsrc(S + 0 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 1>(mkrsrc(S + 1 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 2>(mkrsrc(S + 2 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 3>(mkrsrc(S + 3 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 4>(mkrsrc(S + 4 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 5>(mkrsrc(S + 5 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 6>(mkrsrc(S + 6 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 7>(mkrsrc(S + 7 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 8>(mkrsrc(S + 8 * fs, o.pm), vM, lds0);
-  dma_dword<256u * 9>(mkrsrc(S + 9 * fs, o.pm), vM, lds0);
+  constexpr bool P16 = sizeof(T) == 2 && INLINE;
+  if (P16 && o.pair) {
+    // field pairs (0,1) .. (8,9) into slots 0-4
+    const uint32_t vP = o.m + pair_delta(dp.sx);
+    dma_dword<256u * 0>(mkrsrc(S + 0 * fs, o.pm), vP, lds0);
+    dma_dword<256u * 1>(mkrsrc(S + 2 * fs, o.pm), vP, lds0);
+    dma_dword<256u * 2>(mkrsrc(S + 4 * fs, o.pm), vP, lds0);
+    dma_dword<256u * 3>(mkrsrc(S + 6 * fs, o.pm), vP, lds0);
+    dma_dword<256u * 4>(mkrsrc(S + 8 * fs, o.pm), vP, lds0);
+  } else {
+    dma_dword<256u * 0>(mkrsrc(S + 0 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 1>(mkrsrc(S + 1 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 2>(mkrsrc(S + 2 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 3>(mkrsrc(S + 3 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 4>(mkrsrc(S + 4 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 5>(mkrsrc(S + 5 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 6>(mkrsrc(S + 6 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 7>(mkrsrc(S + 7 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 8>(mkrsrc(S + 8 * fs, o.pm), vM, lds0);
+    dma_dword<256u * 9>(mkrsrc(S + 9 * fs, o.pm), vM, lds0);
+  }
   if constexpr (TWO && !GI) dma_dword<256u * SL::PHI>(mkrsrc((const float*)a.phin + rows * dp.nx, o.pp), o.p, lds0);
   if constexpr (!INLINE) {
     const float* fr = (const float*)a.frc + rows * GF * fs;
@@ -265,39 +289,57 @@ __device__ __forceinline__ void dma_raw(const FusedArgs& a, int zw, size_t fs, c
     }
     if (dp.dens) {
       const T* uo = (const T*)a.uin + rows * 3 * fs;
-      dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * fs, o.pu), vU, lds0);
-      dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * fs, o.pu), vU, lds0);
-      dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * fs, o.pu), vU, lds0);
+      if (P16 && o.pair) {
+        // (u, v) into slot UO, w into the lower half of slot UO+1
+        const uint32_t vP = o.u + pair_delta(dp.sx);
+        dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * fs, o.pu), vP, lds0);
+        if ((threadIdx.x & 63) < 32) dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 2 * fs, o.pu), vP, lds0);
+      } else {
+        dma_dword<256u * (SL::UO + 0)>(mkrsrc(uo + 0 * fs, o.pu), vU, lds0);
+        dma_dword<256u * (SL::UO + 1)>(mkrsrc(uo + 1 * fs, o.pu), vU, lds0);
+        dma_dword<256u * (SL::UO + 2)>(mkrsrc(uo + 2 * fs, o.pu), vU, lds0);
+      }
     }
   }
 }
 
 template <int PREC, bool TWO, bool INLINE, bool GI>
-__device__ __forceinline__ void lds_raw(RawNode<PREC>& r, const float (*slot)[64], int lane, bool dens, uint32_t bT) {
+__device__ __forceinline__ void lds_raw(RawNode<PREC>& r, const float (*slot)[64], int lane, bool dens, uint32_t bT,
+                                        bool pair) {
   typedef Slots<TWO, INLINE, GI> SL;
   typedef typename Codec<PREC>::T T;
-  // a 16-bit element sits at byte (bT & 2) of its lane's dword
+  constexpr bool P16 = sizeof(T) == 2 && INLINE;
+  // a 16-bit element sits at byte (bT & 2) of its lane's dword; in a paired
+  // slot (s0 + f/2) field f's 64 elements are the slot's lower or upper 128 B
   const uint32_t hs = sizeof(T) == 2 ? (bT & 2u) : 0u;
-  auto mom = [&](int f) { return *(const T*)((const char*)&slot[f][lane] + hs); };
+  auto one = [&](int f) { return *(const T*)((const char*)&slot[f][lane] + hs); };
+  auto two = [&](int s0, int f) { return *(const T*)((const char*)&slot[s0 + (f >> 1)][0] + (f & 1) * 128 + lane * 2); };
+  const bool pr = P16 && pair;
+  constexpr bool UO = INLINE && TWO;
+  if (pr) {
 #pragma unroll
-  for (int f = 0; f < 10; ++f) r.m[f] = mom(f);
+    for (int f = 0; f < 10; ++f) r.m[f] = two(0, f);
+    if (UO && dens) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) r.uo[f] = two(SL::UO, f);
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 10; ++f) r.m[f] = one(f);
+    if (UO && dens) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) r.uo[f] = one(SL::UO + f);
+    }
+  }
   if constexpr (TWO && !GI) r.phi = slot[SL::PHI][lane];
   if constexpr (!INLINE) {
 #pragma unroll
     for (int f = 0; f < 3; ++f) r.g[f] = slot[SL::G + f][lane];
   } else if constexpr (GI) {
     r.g[4] = slot[SL::G][lane];
-    if (dens) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) r.uo[f] = mom(SL::UO + f);
-    }
   } else if constexpr (TWO) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) r.g[f] = slot[SL::G + f][lane];
-    if (dens) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) r.uo[f] = mom(SL::UO + f);
-    }
   }
 }
 
@@ -377,6 +419,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   FOff off;
   off.m = ((uint32_t)xw + 10u * srow) * (uint32_t)sizeof(T);
   off.u = ((uint32_t)xw + 3u * srow) * (uint32_t)sizeof(T);
+  // paired 16-bit transfers for row waves over a full 64-node segment
+  // (x0 + 63 < nx: no wrap inside the segment)
+  off.pair = (!halo_wave && XW == 64) ? 1 : 0;
   off.g = ((uint32_t)xw + (uint32_t)GF * srow) * (uint32_t)sizeof(TA);
   off.p = ((uint32_t)xw + (uint32_t)nx * (uint32_t)yw) * (uint32_t)sizeof(TA);
   const uint32_t splane = (uint32_t)sx * (uint32_t)ny;
@@ -500,7 +545,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       if (out_row && lane < XW && zsrc - 2 >= z0) store_lo(zsrc - 2, fs);
 #pragma unroll
       for (int m = 0; m < 10; ++m) { lo[m] = mid[m]; mid[m] = hi[m]; }
-      lds_raw<PREC, TWO, INLINE, GI>(cur, sPF[wave], lane, dp.dens, off.m);
+      lds_raw<PREC, TWO, INLINE, GI>(cur, sPF[wave], lane, dp.dens, off.m, off.pair != 0);
       if constexpr (GI) {
         // the prep's fields 0-3 (prep_body_z, stored at storage precision)
         R cp;
