@@ -294,7 +294,9 @@ __host__ __device__ inline int prep_tiles_z(const DevParams& dp, int tz) {
   return ((dp.nx + 63) / 64) * ((dp.ny + QY - 1) / QY) * ((dp.nz + tz - 1) / tz);
 }
 
-template <int PREC>
+// DO: div theta only (a.div_only); the gradient outputs and the registers
+// that hold them are compiled out
+template <int PREC, bool DO>
 __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile) {
   typedef typename Codec<PREC>::R R;
   typedef typename Aux<PREC>::T TA;
@@ -317,7 +319,7 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   TA* pre = (TA*)a.pre;
   const R FOUR = 4, ONE = 1, G3 = (R)(3.0 / 216.0);
   const R eps = (R)1.0e-9;
-  const bool store_g = !a.div_only;
+  constexpr bool store_g = !DO;
   auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
 
   // theta column of this lane: row waves j = lane + 1 (x = x0 + lane); the
@@ -519,13 +521,13 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   }
 }
 
-template <int PREC>
+template <int PREC, bool DO>
 __global__ void __launch_bounds__(QNT) k_phase_prep_z(StageArgs a, int tz) {
-  prep_body_z<PREC>(a, tz, xcd_tile(blockIdx.x, prep_tiles_z(a.dp, tz)));
+  prep_body_z<PREC, DO>(a, tz, xcd_tile(blockIdx.x, prep_tiles_z(a.dp, tz)));
 }
 
 // many domains per launch (see fused_kernel_batch)
-template <int PREC>
+template <int PREC, bool DO>
 __global__ void __launch_bounds__(QNT)
     k_phase_prep_z_batch(const PrepBatchEntry* __restrict__ tab, int ndom, int total) {
   const int g = xcd_tile(blockIdx.x, total);
@@ -534,7 +536,7 @@ __global__ void __launch_bounds__(QNT)
     const int mid = (lo + hi + 1) >> 1;
     if (tab[mid].tile0 <= g) lo = mid; else hi = mid - 1;
   }
-  prep_body_z<PREC>(tab[lo].a, tab[lo].tz, g - tab[lo].tile0);
+  prep_body_z<PREC, DO>(tab[lo].a, tab[lo].tz, g - tab[lo].tile0);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,7 +935,8 @@ hipError_t launch_stage(int stage, const StageArgs& a, int prec, hipStream_t s) 
 template <int PREC>
 static hipError_t prep_t(const StageArgs& a, int tz, hipStream_t s) {
   if constexpr (sizeof(typename Aux<PREC>::T) == 4) {
-    hipLaunchKernelGGL(k_phase_prep_z<PREC>, dim3(prep_tiles_z(a.dp, tz)), dim3(QNT), 0, s, a, tz);
+    if (a.div_only) hipLaunchKernelGGL((k_phase_prep_z<PREC, true>), dim3(prep_tiles_z(a.dp, tz)), dim3(QNT), 0, s, a, tz);
+    else hipLaunchKernelGGL((k_phase_prep_z<PREC, false>), dim3(prep_tiles_z(a.dp, tz)), dim3(QNT), 0, s, a, tz);
   } else {
     const dim3 grid(((a.dp.nx + PTX - 1) / PTX) * ((a.dp.ny + PTY - 1) / PTY) * ((a.dp.nz + tz - 1) / tz));
     hipLaunchKernelGGL(k_phase_prep<PREC>, grid, dim3(PNT), 0, s, a, tz);
@@ -948,11 +951,13 @@ int prep_tile_count(const StageArgs& a, int prec, int tz) {
 
 hipError_t launch_phase_prep_batch(const PrepBatchEntry* dev_tab, int ndom, int total_tiles, int prec, hipStream_t s) {
   const dim3 grid(total_tiles), block(QNT);
+  // batches hold two-component domains without obstacles: div theta only
+  // exactly where fused_grad_inline() holds, i.e. for every precision but MIXED
   switch (prec) {
-    case PREC_FP32: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_FP32>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
-    case PREC_MIXED: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_MIXED>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
-    case PREC_FP16S: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_FP16S>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
-    case PREC_BF16S: hipLaunchKernelGGL(k_phase_prep_z_batch<PREC_BF16S>, grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_FP32: hipLaunchKernelGGL((k_phase_prep_z_batch<PREC_FP32, true>), grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_MIXED: hipLaunchKernelGGL((k_phase_prep_z_batch<PREC_MIXED, false>), grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_FP16S: hipLaunchKernelGGL((k_phase_prep_z_batch<PREC_FP16S, true>), grid, block, 0, s, dev_tab, ndom, total_tiles); break;
+    case PREC_BF16S: hipLaunchKernelGGL((k_phase_prep_z_batch<PREC_BF16S, true>), grid, block, 0, s, dev_tab, ndom, total_tiles); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
