@@ -278,9 +278,9 @@ __global__ void __launch_bounds__(PNT) k_phase_prep(StageArgs a, int tz) {
 // x0-2+c, wrapped): the row's 64 lanes and two 2-lane transfers for the
 // halo columns stream in by LDS DMA two planes ahead, into a ring of four
 // planes; theta lives in a ring of two planes of 66 columns (x0-1 ..
-// x0+64).  Iteration k reads phi plane k+1 and computes grad/lap/theta of
-// plane k and div theta of plane k-1.  isfluid is read only with internal
-// obstacles.  With a.div_only the gradient and Laplacian are not stored
+// x0+64).  Iteration k computes div theta of plane k-2, then reads phi
+// plane k+1 and computes grad/lap/theta of plane k, with one barrier per
+// plane.  isfluid is read only with internal obstacles.  With a.div_only the gradient and Laplacian are not stored
 // (the fused kernel forms them itself from phi); only div theta is.
 constexpr int QY = 12, QW = 7;
 constexpr int QR = QY + 4;                 // phi rows of a tile (2-row halo)
@@ -376,7 +376,7 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 
   // per theta row h of this lane: phi partials of planes k-1 and k (pm, p0)
   // and phi at the node in plane k (c0); per div row: theta partials of
-  // planes k-2 and k-1 (em, e0)
+  // planes k-3 and k-2 (em, e0)
   Part3<R> pm[2], p0[2], em[2], e0[2];
   R c0[2];
   // outputs held for one iteration (see below)
@@ -386,8 +386,12 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 
   dma_plane(z0 - 2);
   dma_plane(z0 - 1);
-  // iterations z0-3 and z0-2 only fill the partials of planes z0-2, z0-1
-  for (int k = z0 - 3; k <= zend; ++k) {
+  // Iteration k: div theta of plane k-2 from the theta of planes k-3 .. k-1,
+  // then phi partials of plane k+1 and theta of plane k.  One barrier per
+  // plane: theta(k-1), written before it, is read after it; theta(k) goes to
+  // the other slot of the two-plane ring.  Iterations z0-3 and z0-2 only fill
+  // the partials of planes z0-2, z0-1; iteration zend+1 only finishes the div.
+  for (int k = z0 - 3; k <= zend + 1; ++k) {
     // phi(k+1) has landed once only the newest plane's 3*QDMA transfers are outstanding
     if (k + 2 <= zend + 1)
       asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * QDMA) : "memory");
@@ -411,11 +415,53 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
       st_d[h] = false;
     }
 
+    // ---- theta partials of plane k-1; div theta of plane k-2 on theta rows
+    // 1 + wave, 1 + wave + QW -----------------------------------------------------------
+    const int zo = k - 2;
+    if (!edge_wave && k - 1 >= z0 - 1 && k - 1 <= zend) {
+      const int tk = (k - 1) & 1;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int t = 1 + wave + QW * h;
+        if (t > QY) continue;
+        R w[3][3];
+        Part3<R> ep;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][0][t - 1 + bb][tj - 1 + aa];
+        ep.dx = (w[0][2] - w[0][0]) + FOUR * (w[1][2] - w[1][0]) + (w[2][2] - w[2][0]);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][1][t - 1 + bb][tj - 1 + aa];
+        ep.dy = (w[2][0] - w[0][0]) + FOUR * (w[2][1] - w[0][1]) + (w[2][2] - w[0][2]);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][2][t - 1 + bb][tj - 1 + aa];
+        ep.s = (w[0][0] + FOUR * w[0][1] + w[0][2]) + FOUR * (w[1][0] + FOUR * w[1][1] + w[1][2]) +
+               (w[2][0] + FOUR * w[2][1] + w[2][2]);
+        if (zo >= z0 && zo < zend) {
+          const R d = G3 * ((em[h].dx + FOUR * e0[h].dx + ep.dx) + (em[h].dy + FOUR * e0[h].dy + ep.dy) +
+                            (ep.s - em[h].s));
+          const int y = y0 - 1 + t;
+          if (out_lane && y < ny && fluid(xo, y, zo)) {
+            hd[h] = d;
+            od[h] = sbase(5, xo, y, zo, sx, ny);
+            st_d[h] = true;
+          }
+        }
+        em[h] = e0[h];
+        e0[h] = ep;
+      }
+    }
+
     // ---- phi partials of plane k+1; grad / lap / theta of plane k -----------------
     // row waves: theta rows wave, wave + QW (columns 1..64); edge wave: rows
     // lane >> 1 at columns 0 and 65
     static_assert(QT == 2 * QW, "two theta rows per wave");
-    {
+    if (k + 1 <= zend + 1) {
       const int sp = (k + 1) & 3, tk = k & 1;
       const bool have_theta = k >= z0 - 1;
       const bool own = k >= z0 && k < zend;
@@ -461,49 +507,6 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
         pm[h] = p0[h];
         p0[h] = pp;
         c0[h] = cp;
-      }
-    }
-    __syncthreads();
-
-    // ---- theta partials of plane k; div theta of plane k-1 on theta rows 1 + wave,
-    // 1 + wave + QW ------------------------------------------------------------------
-    const int zo = k - 1;
-    if (!edge_wave && k >= z0 - 1) {
-      const int tk = k & 1;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int t = 1 + wave + QW * h;
-        if (t > QY) continue;
-        R w[3][3];
-        Part3<R> ep;
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][0][t - 1 + bb][tj - 1 + aa];
-        ep.dx = (w[0][2] - w[0][0]) + FOUR * (w[1][2] - w[1][0]) + (w[2][2] - w[2][0]);
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][1][t - 1 + bb][tj - 1 + aa];
-        ep.dy = (w[2][0] - w[0][0]) + FOUR * (w[2][1] - w[0][1]) + (w[2][2] - w[0][2]);
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-          for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][2][t - 1 + bb][tj - 1 + aa];
-        ep.s = (w[0][0] + FOUR * w[0][1] + w[0][2]) + FOUR * (w[1][0] + FOUR * w[1][1] + w[1][2]) +
-               (w[2][0] + FOUR * w[2][1] + w[2][2]);
-        if (zo >= z0 && zo < zend) {
-          const R d = G3 * ((em[h].dx + FOUR * e0[h].dx + ep.dx) + (em[h].dy + FOUR * e0[h].dy + ep.dy) +
-                            (ep.s - em[h].s));
-          const int y = y0 - 1 + t;
-          if (out_lane && y < ny && fluid(xo, y, zo)) {
-            hd[h] = d;
-            od[h] = sbase(5, xo, y, zo, sx, ny);
-            st_d[h] = true;
-          }
-        }
-        em[h] = e0[h];
-        e0[h] = ep;
       }
     }
     // slot (k+3)&3 held plane k-1, last read in iteration k-2
