@@ -250,16 +250,16 @@ template <> struct Codec<PREC_FP16S> {
   static constexpr bool dev = true;
   __device__ static R dec(T v, float inv) { return __half2float(v) * inv; }
   __device__ static T enc(R v, float s) { return __float2half_rn(v * s); }
-  // stochastic rounding of the scaled value x: x moved by a uniform offset in
-  // [-1/2, 1/2) half ulp (13 random bits below the half significand, in the
-  // normal range where every stored scaled deviation sits), then rounded to
-  // nearest, which rounds up with probability equal to the dropped fraction
+  // stochastic rounding of the scaled value x: 13 random bits (a uniform
+  // offset in [0, 1) ulp of the half significand, in the normal range where
+  // every stored scaled deviation sits) added to the magnitude, the bits
+  // below the half significand cleared, then an exact conversion: the value
+  // rounds away from zero with probability equal to the dropped fraction.
+  // The sign bit is untouched (the add cannot carry into it for finite x),
+  // and x = 0 stays 0.  Below the half normal range the conversion rounds
+  // the dithered value to nearest.
   __device__ static T srx(float x, uint32_t r) {
-    // the offset acts on the magnitude bits, clamped at zero (|x| below
-    // 2^-137 rounds to 0 either way)
-    const uint32_t b = __float_as_uint(x);
-    const int m = (int)(b & 0x7FFFFFFFu) + (int)(r & 0x1FFFu) - 0x1000;
-    return __float2half_rn(__uint_as_float((b & 0x80000000u) | (uint32_t)(m > 0 ? m : 0)));
+    return __float2half_rn(__uint_as_float((__float_as_uint(x) + (r & 0x1FFFu)) & ~0x1FFFu));
   }
   __device__ static R round(R v) { return v; }
 };
