@@ -37,6 +37,15 @@
 // recomputation is only the two y-halo rows, the x-halo wave and the two
 // z-halo planes of the march.
 //
+// Phase-field inputs (two-component, inline variant).  Without internal
+// obstacles and with float phase-field storage the block also streams a phi
+// tile with a 2-node halo and forms grad phi and lap phi of every collided
+// node from the separable partial sums of three planes (part3 / grad_lap,
+// the arithmetic of the prep kernel), so the prep only stores div theta.
+// With obstacles, PHI_int_boundary (lb_cuda_boundary_module.f90:218-459)
+// rewrites phi on solid nodes after compute_norm and before the moments
+// stage, so the gradients come from the prep's output instead.
+//
 // Periodicity: indices wrap in-kernel (the reference's intpbc halo copies,
 // mpi_module.f90:2820-2918).  isfluid is read from a copy with the
 // reference's 1-node halo (periodic image or 1, LBFAST.f90:258-328).
