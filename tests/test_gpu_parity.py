@@ -161,7 +161,7 @@ def test_tile_shapes(tile):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("n", [(96, 20, 10), (16, 40, 12), (130, 9, 7)])
+@pytest.mark.parametrize("n", [(96, 20, 10), (16, 40, 12), (130, 9, 7), (37, 5, 3), (65, 7, 2)])
 def test_two_component_tiling_edges_fp32(n):
     """The FP32 two-component fast path without obstacles, where the fused
     kernel forms grad phi and lap phi from its own phi tile and the prep

@@ -83,7 +83,9 @@ def _droplet(n, prec):
     return p, f
 
 
-@pytest.mark.parametrize("n", [(32, 32, 32), (33, 17, 21)])
+# (96, 20, 12) and (130, 9, 7): full 64-node row segments (two 16-bit fields
+# per LDS-DMA transfer) beside short ones (one field per transfer) in one domain
+@pytest.mark.parametrize("n", [(32, 32, 32), (33, 17, 21), (96, 20, 12), (130, 9, 7)])
 @pytest.mark.parametrize("prec,tol_u", [(PREC_FP16S, 3e-3), (PREC_BF16S, 3e-2)])
 def test_sixteen_bit_steps_against_fp64(n, prec, tol_u):
     steps = 50
