@@ -206,9 +206,16 @@ int32_t lbf_get_step(const lbf_ctx* ctx);
  * up to even and F = 10, 3, 3, 7, 6 respectively. */
 void* lbf_field_device_ptr(lbf_ctx* ctx, int field_id);
 
-/* Timing of the last lbf_step call's kernels, measured with HIP events on
- * the context stream: total ms and ms of the dominant (fused) kernel. */
+/* Timing of the last lbf_step call, measured with HIP events on the context
+ * stream: total ms of the call, and the summed ms of its fused-kernel
+ * launches when per-launch timing is on (else fused_ms = -1).  No
+ * counterpart in the reference beyond its host timers (profiling_m.f90). */
 int lbf_last_step_timing(lbf_ctx* ctx, double* total_ms, double* fused_ms);
+/* Per-launch timing (off by default): when on, every fused launch of
+ * lbf_step / lbf_stage is bracketed by its own HIP event pair, which costs
+ * two event records per step and one event pair per step of the longest
+ * call.  Profilers and benchmarks turn it on; long production runs leave it off. */
+int lbf_set_timing(lbf_ctx* ctx, int32_t on);
 
 /* Fused-kernel tile shape override (0 = default/autotuned). */
 int lbf_set_tile(lbf_ctx* ctx, int32_t tile_y, int32_t tile_z);

@@ -51,8 +51,9 @@ struct lbf_ctx {
   bool isf_set = false;
   bool raw_pending = false;
   int ty = 8, tz = 0, tz_prep = 64;
-  std::vector<hipEvent_t> evs;
+  std::vector<hipEvent_t> evs;  // per-launch event pairs (lbf_set_timing)
   int nev_used = 0;
+  bool time_launches = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   bool timing_valid = false;
   std::vector<void*> allocs;
@@ -271,6 +272,22 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   if (p.visc1 <= 0 || (p.twocomponent && p.visc2 <= 0)) return fail(nullptr, LBF_EINVAL, "viscosity must be > 0");
   if (p.twocomponent && p.densratio && (p.rho_r <= 0 || p.rho_b <= 0)) return fail(nullptr, LBF_EINVAL, "rho_r, rho_b must be > 0");
   if (lbf_params_derive(&p) != LBF_OK) return fail(nullptr, LBF_EINVAL, "width must be non-zero for TWOCOMPONENT");
+  {
+    // The kernels address one z plane of a field set with 32-bit byte offsets
+    // and 32-bit buffer-resource ranges: the widest plane (the 10-field moment
+    // set, or the 7-field aux set, or isfluid with its halo) must stay below 4 GiB.
+    const uint64_t sx = (uint64_t)set_sx(p.nx), ny = (uint64_t)p.ny;
+    const uint64_t tsz = p.precision == LBF_PREC_FP64 ? 8 : (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S) ? 2 : 4;
+    const uint64_t asz = p.precision == LBF_PREC_FP64 ? 8 : 4;
+    uint64_t plane = 10 * sx * ny * tsz;
+    if (7 * sx * ny * asz > plane) plane = 7 * sx * ny * asz;
+    if ((uint64_t)(p.nx + 2) * (uint64_t)(p.ny + 2) > plane) plane = (uint64_t)(p.nx + 2) * (uint64_t)(p.ny + 2);
+    if (plane + 4 > 0xFFFFFFFFull)
+      return fail(nullptr, LBF_ESIZE,
+                  "an x-y plane of %dx%d needs %llu bytes per field set, beyond the kernels' 32-bit plane offsets "
+                  "(at most 4 GiB per plane; split the domain along x or y)",
+                  p.nx, p.ny, (unsigned long long)plane);
+  }
 
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, LBF_ENODEV, "no HIP device");
@@ -545,6 +562,8 @@ static FusedArgs fused_args(lbf_ctx* c) {
   a.dp = c->dp;
   a.ty = c->ty;
   a.tz = c->tz;
+  a.seed = (uint32_t)c->step;
+  a.seedp = nullptr;
   return a;
 }
 
@@ -586,6 +605,10 @@ int lbf_init_state(lbf_ctx* c) {
 
 static int run_fused(lbf_ctx* c, bool inl) {
   FusedArgs a = fused_args(c);
+  if (!c->time_launches) {
+    HIPCHK(c, launch_fused(a, c->prec, inl, c->stream));
+    return LBF_OK;
+  }
   if (c->nev_used + 2 > (int)c->evs.size()) {
     for (int i = 0; i < 2; ++i) {
       hipEvent_t ev;
@@ -664,6 +687,10 @@ int lbf_last_step_timing(lbf_ctx* c, double* total_ms, double* fused_ms) {
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
   *total_ms = ms;
+  if (!c->time_launches) {
+    *fused_ms = -1.0;  // per-launch timing is off
+    return LBF_OK;
+  }
   double f = 0;
   for (int i = 0; i + 1 < c->nev_used; i += 2) {
     float m2 = 0;
@@ -671,6 +698,14 @@ int lbf_last_step_timing(lbf_ctx* c, double* total_ms, double* fused_ms) {
     f += m2;
   }
   *fused_ms = f;
+  return LBF_OK;
+}
+
+int lbf_set_timing(lbf_ctx* c, int32_t on) {
+  if (!c) return LBF_EINVAL;
+  c->time_launches = on != 0;
+  c->nev_used = 0;
+  c->timing_valid = false;
   return LBF_OK;
 }
 
@@ -756,6 +791,8 @@ struct lbf_batch {
   PrepBatchEntry* ptab[2] = {};
   int ftotal[2] = {}, ptotal[2] = {};
   int par = 0;  // 0: tables built from the contexts' current buffers
+  std::vector<int> tcur, tucur, tstep;  // each context's (cur, ucur, step) that table 0 describes
+  uint32_t* ctr = nullptr;  // per-domain step counters: the 16-bit codecs' rounding seeds
   hipStream_t stream = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   std::vector<hipEvent_t> cev;  // one per context: orders its stream before batch work
@@ -804,6 +841,10 @@ static int batch_tables(lbf_batch* b) {
       f[d].a = fused_args(c);
       f[d].a.ty = b->ty;
       f[d].a.tz = b->tz[d];
+      // the seed of a step is the domain's step number, as in lbf_step:
+      // 1 + the counter, which the step's tick advances afterwards
+      f[d].a.seed = 1u;
+      f[d].a.seedp = b->ctr + d;
       f[d].tile0 = ft;
       ft += fused_tile_count(f[d].a);
       q[d].a = stage_args(c);
@@ -821,7 +862,32 @@ static int batch_tables(lbf_batch* b) {
     b->ftotal[par] = ft;
     b->ptotal[par] = pt;
   }
+  b->tcur.resize(nd);
+  b->tucur.resize(nd);
+  b->tstep.resize(nd);
+  std::vector<uint32_t> steps(nd);
+  for (int d = 0; d < nd; ++d) {
+    b->tcur[d] = b->ctx[d]->cur;
+    b->tucur[d] = b->ctx[d]->ucur;
+    b->tstep[d] = b->ctx[d]->step;
+    steps[d] = (uint32_t)b->ctx[d]->step;
+  }
+  BHIPCHK(b, hipMemcpy(b->ctr, steps.data(), nd * sizeof(uint32_t), hipMemcpyHostToDevice));
+  b->par = 0;
   return LBF_OK;
+}
+
+// A member context stepped on its own (lbf_step, the stage path) since the
+// tables were built has swapped its buffers: rebuild the tables from the
+// contexts' current state and drop a captured graph that refers to the old ones.
+static int batch_refresh(lbf_batch* b) {
+  bool stale = false;
+  for (size_t d = 0; d < b->ctx.size(); ++d)
+    stale |= b->ctx[d]->cur != b->tcur[d] || b->ctx[d]->ucur != b->tucur[d] || b->ctx[d]->step != b->tstep[d];
+  if (!stale) return LBF_OK;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  return batch_tables(b);
 }
 
 // Enqueue one step on b->stream at the batch's current parity (no state change).
@@ -829,6 +895,7 @@ static int batch_enqueue(lbf_batch* b, int par) {
   const int nd = (int)b->ctx.size();
   if (b->two) BHIPCHK(b, launch_phase_prep_batch(b->ptab[par], nd, b->ptotal[par], b->prec, b->stream));
   BHIPCHK(b, launch_fused_batch(b->ftab[par], nd, b->ftotal[par], b->prec, b->lattice, b->ho, b->two, b->ty, b->stream));
+  if (b->prec == LBF_PREC_FP16S || b->prec == LBF_PREC_BF16S) BHIPCHK(b, launch_tick(b->ctr, nd, b->stream));
   return LBF_OK;
 }
 
@@ -872,7 +939,7 @@ int lbf_batch_create(lbf_ctx* const* ctxs, int32_t ndomains, lbf_batch** out) {
     b->tz_prep.push_back(c->p.nz < 32 ? c->p.nz : 32);
   }
   bool ok = hipStreamCreate(&b->stream) == hipSuccess && hipEventCreate(&b->t0) == hipSuccess &&
-            hipEventCreate(&b->t1) == hipSuccess;
+            hipEventCreate(&b->t1) == hipSuccess && hipMalloc(&b->ctr, ndomains * sizeof(uint32_t)) == hipSuccess;
   for (lbf_ctx* c : b->ctx) {
     hipEvent_t e = nullptr;
     ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
@@ -901,6 +968,7 @@ void lbf_batch_destroy(lbf_batch* b) {
     if (b->ftab[i]) hipFree(b->ftab[i]);
     if (b->ptab[i]) hipFree(b->ptab[i]);
   }
+  if (b->ctr) hipFree(b->ctr);
   if (b->t0) hipEventDestroy(b->t0);
   if (b->t1) hipEventDestroy(b->t1);
   for (hipEvent_t e : b->cev)
@@ -931,6 +999,10 @@ int lbf_batch_set_tile(lbf_batch* b, int32_t ty, int32_t tz, int32_t tz_prep) {
 int lbf_batch_step(lbf_batch* b, int32_t nsteps) {
   if (!b) return LBF_EINVAL;
   if (nsteps < 0) return bfail(b, LBF_EINVAL, "nsteps < 0");
+  {
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+  }
   // work queued on a domain's own stream (uploads) comes first ...
   for (size_t d = 0; d < b->ctx.size(); ++d) {
     BHIPCHK(b, hipEventRecord(b->cev[d], b->ctx[d]->stream));
@@ -963,6 +1035,11 @@ int lbf_batch_step(lbf_batch* b, int32_t nsteps) {
     b->par = 0;
     batch_drop_graph(b);  // a captured graph refers to the old parity's tables
   }
+  for (size_t d = 0; d < b->ctx.size(); ++d) {
+    b->tcur[d] = b->ctx[d]->cur;
+    b->tucur[d] = b->ctx[d]->ucur;
+    b->tstep[d] = b->ctx[d]->step;
+  }
   return LBF_OK;
 }
 
@@ -970,6 +1047,10 @@ int lbf_batch_capture(lbf_batch* b, int32_t nsteps) {
   if (!b) return LBF_EINVAL;
   if (nsteps < 2 || (nsteps & 1)) return bfail(b, LBF_EINVAL, "a captured graph holds an even number of steps (>= 2)");
   BHIPCHK(b, hipStreamSynchronize(b->stream));
+  {
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+  }
   batch_drop_graph(b);
   BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
@@ -1014,6 +1095,10 @@ int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int3
   if (!b) return LBF_EINVAL;
   if (trial_steps < 2 || (trial_steps & 1)) return bfail(b, LBF_EINVAL, "trial_steps must be even and >= 2");
   BHIPCHK(b, hipStreamSynchronize(b->stream));
+  {
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+  }
   for (lbf_ctx* c : b->ctx) BHIPCHK(b, hipStreamSynchronize(c->stream));
   // snapshot: the S moments, u_old and phi each step reads
   std::vector<void*> snap;

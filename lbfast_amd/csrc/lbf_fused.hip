@@ -449,7 +449,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   T* Sout = (T*)a.mout;
   const size_t plM = (size_t)10 * sx * ny;  // elements of one z plane of the moment set
   // 16-bit codecs: per-node seed of the stochastic rounding
-  const uint32_t nseed = C::dev ? mix32((uint32_t)xw * 0x9E3779B9u ^ (uint32_t)(yw + 1) * 0x85EBCA6Bu) : 0u;
+  // and of the step: without a step term a node whose sibling field is
+  // constant (e.g. exactly 0 in a planar flow) would draw the same threshold
+  // every step, which makes its rounding deterministic and biased
+  uint32_t nseed = 0u;
+  if constexpr (C::dev) {
+    const uint32_t step = a.seed + (a.seedp ? *a.seedp : 0u);
+    nseed = mix32((uint32_t)xw * 0x9E3779B9u ^ (uint32_t)(yw + 1) * 0x85EBCA6Bu ^ mix32(step * 0x632BE5ABu + 0x1B873593u));
+  }
 
   R lo[10], mid[10], hi[10];
 #pragma unroll

@@ -18,6 +18,10 @@ struct FusedArgs {
   const int8_t* isfh;  // isfluid with 1-node halo
   DevParams dp;
   int ty, tz;        // tile rows, z-chunk length
+  // 16-bit storage: the step's stochastic-rounding seed is seed (+ *seedp
+  // when seedp is set: a device counter that captured graphs advance)
+  uint32_t seed;
+  const uint32_t* seedp;
 };
 
 // Stage-path buffers (the reference's per-kernel arguments), as field sets.
@@ -91,6 +95,8 @@ hipError_t launch_set_copy(void* dst, FieldLay dl, const void* src, FieldLay sl,
 hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                          double* partial, int nblocks, double* out8, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+// ctr[d] += 1 for d < n (the batch's per-domain step counters)
+hipError_t launch_tick(uint32_t* ctr, int n, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // element kinds for convert: 0 double, 1 float, 2 half (scaled), 3 bf16 (scaled)

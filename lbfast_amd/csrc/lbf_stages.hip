@@ -1116,6 +1116,17 @@ hipError_t launch_set_copy(void* dst, FieldLay dl, const void* src, FieldLay sl,
   return hipGetLastError();
 }
 
+__global__ void k_tick(uint32_t* ctr, int n) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < n) ctr[d] += 1u;
+}
+
+hipError_t launch_tick(uint32_t* ctr, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctr, n);
+  return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(BS) k_fill_zero(uint32_t* p, size_t nwords) {
   const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
   if (i < nwords) p[i] = 0u;

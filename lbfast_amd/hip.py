@@ -25,7 +25,7 @@ EXPORTS = (
     "lbf_upload_field", "lbf_download_field", "lbf_upload_isfluid", "lbf_init_state",
     "lbf_step", "lbf_begin_step", "lbf_stage", "lbf_end_step", "lbf_reduce_fp64",
     "lbf_synchronize", "lbf_get_step", "lbf_field_device_ptr", "lbf_last_step_timing",
-    "lbf_set_tile",
+    "lbf_set_tile", "lbf_set_timing",
     "lbf_batch_create", "lbf_batch_destroy", "lbf_batch_last_error", "lbf_batch_step", "lbf_batch_capture",
     "lbf_batch_set_tile", "lbf_batch_autotune", "lbf_batch_synchronize", "lbf_batch_timing",
 )
@@ -69,6 +69,7 @@ def load(path: str = LIB_PATH):
     L.lbf_field_device_ptr.restype = vp
     L.lbf_last_step_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.lbf_set_tile.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    L.lbf_set_timing.argtypes = [vp, ctypes.c_int32]
     i32, pi32 = ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)
     L.lbf_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]
     L.lbf_batch_destroy.argtypes = [vp]
@@ -187,7 +188,13 @@ class Domain:
     def set_tile(self, ty: int = 0, tz: int = 0):
         self._chk(self.L.lbf_set_tile(self.h, ty, tz), "set_tile")
 
+    def set_timing(self, on: bool = True):
+        """Bracket every fused launch with a HIP event pair (lbf_set_timing)."""
+        self._chk(self.L.lbf_set_timing(self.h, 1 if on else 0), "set_timing")
+
     def timing(self):
+        """(total ms, fused-kernel ms) of the last step call; the second is -1
+        unless set_timing(True) was called."""
         t, f = ctypes.c_double(), ctypes.c_double()
         self._chk(self.L.lbf_last_step_timing(self.h, ctypes.byref(t), ctypes.byref(f)), "timing")
         return t.value, f.value

@@ -77,6 +77,36 @@ def test_create_rejects_bad_switches():
         assert L.lbf_create(ctypes.byref(p), ctypes.byref(h)) == 1
 
 
+@pytest.mark.parametrize("n,prec,ok", [
+    ((8192, 8192, 8), 0, False),    # FP64: 10 * 8192 * 8192 * 8 B per plane = 5.4 GB
+    ((16384, 8192, 2), 1, False),   # FP32: 10 * 16384 * 8192 * 4 B = 5.4 GB
+    ((8192, 8192, 2), 3, True),     # FP16S: 10 * 8192 * 8192 * 2 B = 1.34 GB; aux 7 * 4 B: 1.9 GB
+    ((4096, 4096, 2), 0, True),     # FP64: 1.34 GB per plane
+])
+def test_create_rejects_planes_beyond_32bit_offsets(n, prec, ok):
+    # The kernels address one x-y plane of a field set with 32-bit byte
+    # offsets; lbf_create refuses a plane of 4 GiB or more before it touches
+    # the device, so this runs without a GPU.
+    L = hip.load()
+    p = make_params(n, precision=prec)
+    h = ctypes.c_void_p()
+    if ok:
+        # accepted sizes would allocate gigabytes on a GPU: check them only
+        # where lbf_create stops at the missing device
+        rc = L.lbf_create(ctypes.byref(p), ctypes.byref(h))
+        if rc == 0:
+            L.lbf_destroy(h)
+            pytest.skip("GPU present")
+        assert rc in (3, 4), L.lbf_global_last_error()
+        return
+    rc = L.lbf_create(ctypes.byref(p), ctypes.byref(h))
+    if rc == 0:
+        L.lbf_destroy(h)
+    else:
+        assert rc == 2 and not h.value
+        assert b"32-bit plane offsets" in L.lbf_global_last_error()
+
+
 def test_namelist_reader():
     from lbfast_amd.params import read_namelists, params_from_namelist
     deck = """&simulation

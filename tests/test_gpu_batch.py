@@ -166,3 +166,76 @@ def test_batch_rejects_mixed_configurations():
         hip.Batch([d1, d2])
     d1.close()
     d2.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16S])
+def test_batch_interleaved_with_single_domain_steps(prec):
+    # A member stepped on its own between batch calls swaps its buffers and
+    # advances its step count; the next batch call rebuilds its tables (and
+    # drops a captured graph) from the contexts' current state.
+    print(f"[batch] interleaved single-domain steps prec={prec}", flush=True)
+    doms = _domains(SPECS[:3], prec)
+    want = []
+    for p, f in doms:
+        d = hip.Domain(p)
+        d.load(f)
+        d.step(6)
+        want.append(_state(d, p))
+        d.close()
+    ds = []
+    for p, f in doms:
+        d = hip.Domain(p)
+        d.load(f)
+        ds.append(d)
+    b = hip.Batch(ds)
+    b.capture(2)
+    b.step(3)
+    ds[1].step(1)            # one member alone: odd, so its buffers swap
+    b.step(2)                # the others have 5 steps, member 1 has 6
+    ds[0].step(1)
+    ds[2].step(1)            # now all have 6
+    b.synchronize()
+    for (p, _), d, w in zip(doms, ds, want):
+        assert d.step_count == 6
+        got = _state(d, p)
+        assert np.array_equal(got, w), float(np.abs(got - w).max())
+    b.close()
+    for d in ds:
+        d.close()
+
+
+def test_mixed_size_batch_against_oracle():
+    # Domains of three shapes advanced together, each compared with the FP32
+    # oracle (a CPU port of the reference stages) of its own problem: the
+    # batch is held to the reference's arithmetic, not only to single-domain
+    # runs.  Bar: the FP32 parity bar of the single-domain tests -- rho_phi,
+    # u and phi within 1e-5 relative L2 of the same-precision oracle.
+    import oracle
+    from tests.cases import rel_l2
+    specs = [(32, 32, 32), (48, 48, 48), (40, 24, 32)]
+    steps = 12
+    print(f"[batch] mixed sizes vs oracle {specs} steps={steps}", flush=True)
+    doms = _domains(specs, PREC_FP32)
+    ds = []
+    for p, f in doms:
+        d = hip.Domain(p)
+        d.load(f)
+        ds.append(d)
+    b = hip.Batch(ds)
+    b.step(steps)
+    b.synchronize()
+    for (p, f), d in zip(doms, ds):
+        o = oracle.Oracle(p)
+        o.load(f)
+        o.step(steps)
+        phi_g, phi_o = d.download(FIELD_PHI), o.get_field(FIELD_PHI)
+        err = {"u": rel_l2(np.stack([d.download(i) for i in (2, 3, 4)]), np.stack([o.get_field(i) for i in (2, 3, 4)])),
+               "phi": rel_l2(phi_g, phi_o),
+               "rho": rel_l2(p.rho_r * phi_g + (1 - phi_g) * p.rho_b, p.rho_r * phi_o + (1 - phi_o) * p.rho_b),
+               "uold": rel_l2(np.stack([d.download(i) for i in FIELD_UOLD]),
+                              np.stack([o.get_field(i) for i in FIELD_UOLD]))}
+        print(f"[batch]   {p.nx}x{p.ny}x{p.nz}: {err}", flush=True)
+        assert max(err.values()) < 1e-5, err
+    b.close()
+    for d in ds:
+        d.close()

@@ -133,3 +133,33 @@ def test_porous_channel_against_fp64():
         assert np.all(np.isfinite(u))
         assert eu < tol, (prec, eu)
         assert ephi < 1e-3, (prec, ephi)
+
+
+def test_planar_channel_fp16s_rounding_varies_by_step():
+    """A planar Poiseuille channel (walls in y, body force in x): w, Pxz and
+    Pyz are exactly 0 at every node, so the siblings whose low bits feed the
+    stochastic rounding of v, Pxy and Pxz are constant.  The rounding seed
+    also mixes in the step number, so each store still draws new bits every
+    step: the FP16S velocity stays within the 1e-3 bar of the FP64 run after
+    the flow has developed (4000 steps, about 2.6 diffusion times)."""
+    n, steps = (16, 18, 4), 4000
+    res = {}
+    for prec in (PREC_FP64, PREC_FP16S):
+        print(f"[precision] planar channel prec={prec} n={n} steps={steps}", flush=True)
+        p = make_params(n, lattice=27, highorder=True, internal_obstacles=True, pbc=(1, 0, 1), visc1=1.0 / 6.0,
+                        fx=5e-6, precision=prec)
+        z = np.zeros((n[2], n[1], n[0]))
+        f = {k: z.copy() for k in HFIELD_NAMES}
+        isf = hip.classify_isfluid(p, None)
+        d = hip.Domain(p)
+        d.load(f, isf)
+        d.step(steps)
+        res[prec] = (np.stack([d.download(i) for i in (2, 3, 4)]), isf)
+        d.close()
+    (u64, isf), (u16, _) = res[PREC_FP64], res[PREC_FP16S]
+    fl = isf != 0
+    eu = np.linalg.norm((u16 - u64)[:, fl]) / np.linalg.norm(u64[:, fl])
+    umax = float(np.abs(u64[0]).max())
+    print(f"[precision]   planar channel u_max {umax:.3e}, FP16S velocity rel-L2 {eu:.2e}", flush=True)
+    assert umax > 5e-4
+    assert eu < 1e-3
