@@ -151,12 +151,19 @@ __device__ __forceinline__ void lds_wait_all() { asm volatile("s_waitcnt lgkmcnt
 //   DX(F) = sum_c u_c sum_b u_b (F(1,b,c) - F(-1,b,c)) / 216   (DY, DZ alike)
 //   LAPL  = 6 (sum_abc w(a,b,c) F(a,b,c) - F(0,0,0)).
 // One z plane contributes three 9-point partial sums per node; the z
-// direction combines the partials of three consecutive planes:
-//   grad = 3/216 (dx_m + 4 dx_0 + dx_p,  dy_m + 4 dy_0 + dy_p,  s_p - s_m)
-//   lap  = 6/216 (s_m + 4 s_0 + s_p) - 6 F(0,0,0)
-// the three partial sums of one plane at one node
+// direction combines the partials of three consecutive planes m, 0, p.
+// The 2-D smoothing s is taken of the differences to the plane's own centre
+// value c (which is kept beside it): summing the 36-weight smoothing of O(1)
+// values and cancelling afterwards, as s_p - s_m and s_m + 4 s_0 + s_p - 216 c
+// do, loses about six bits in FP32 at a smooth interface; differences first
+// keep the gradient as accurate as the reference's paired differences and the
+// Laplacian more accurate than its weighted sum.  With S = s + 36 c:
+//   grad = 3/216 (dx_m + 4 dx_0 + dx_p,  dy_m + 4 dy_0 + dy_p,  s_p - s_m) + (0, 0, (c_p - c_m)/2)
+//   lap  = 6/216 (s_m + 4 s_0 + s_p) + (c_m - c_0) + (c_p - c_0)
+// the partial sums of one plane at one node
 template <class R> struct Part3 {
-  R dx, dy, s;  // x difference smoothed in y, y difference smoothed in x, 2-D smoothing
+  R dx, dy;  // x difference smoothed in y, y difference smoothed in x
+  R s, c;    // 2-D smoothing of (value - c), and c = the value at the node's column
 };
 
 // partials of the 3 x 3 neighbourhood v[b][a] (b: y offset + 1, a: x offset + 1)
@@ -166,23 +173,25 @@ __device__ __forceinline__ Part3<R> part3(const R (&v)[3][3]) {
   Part3<R> q;
   q.dx = (v[0][2] - v[0][0]) + FOUR * (v[1][2] - v[1][0]) + (v[2][2] - v[2][0]);
   q.dy = (v[2][0] - v[0][0]) + FOUR * (v[2][1] - v[0][1]) + (v[2][2] - v[0][2]);
-  const R r0 = v[0][0] + FOUR * v[0][1] + v[0][2];
-  const R r1 = v[1][0] + FOUR * v[1][1] + v[1][2];
-  const R r2 = v[2][0] + FOUR * v[2][1] + v[2][2];
+  const R c = v[1][1];
+  const R r0 = (v[0][0] - c) + FOUR * (v[0][1] - c) + (v[0][2] - c);
+  const R r1 = (v[1][0] - c) + (v[1][2] - c);
+  const R r2 = (v[2][0] - c) + FOUR * (v[2][1] - c) + (v[2][2] - c);
   q.s = r0 + FOUR * r1 + r2;
+  q.c = c;
   return q;
 }
 
 // gradient and Laplacian at a node from the partials of planes z-1, z, z+1
-// (m, o, p) and the node's own value c
+// (m, o, p); o.c is the node's own value
 template <class R>
-__device__ __forceinline__ void grad_lap(const Part3<R>& m, const Part3<R>& o, const Part3<R>& p, R c, R& gx, R& gy,
+__device__ __forceinline__ void grad_lap(const Part3<R>& m, const Part3<R>& o, const Part3<R>& p, R& gx, R& gy,
                                          R& gz, R& lap) {
-  const R FOUR = 4, SIX = 6, G3 = (R)(3.0 / 216.0), L6 = (R)(6.0 / 216.0);
+  const R FOUR = 4, HALF = 0.5, G3 = (R)(3.0 / 216.0), L6 = (R)(6.0 / 216.0);
   gx = G3 * (m.dx + FOUR * o.dx + p.dx);
   gy = G3 * (m.dy + FOUR * o.dy + p.dy);
-  gz = G3 * (p.s - m.s);
-  lap = L6 * (m.s + FOUR * o.s + p.s) - SIX * c;
+  gz = G3 * (p.s - m.s) + HALF * (p.c - m.c);
+  lap = L6 * (m.s + FOUR * o.s + p.s) + ((m.c - o.c) + (p.c - o.c));
 }
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
@@ -419,12 +428,18 @@ __host__ __device__ constexpr double weight(int p, int q, int r) {
 // f(c) = sum_a g_a(c) A_a with g_0 = w1d(c), g_1 = 3 w1d c,
 // g_2 = 4.5 w1d (c^2 - 1/3), w1d = (1/6, 2/3, 1/6).  (Minv_mrt of
 // vars_module.f90:359-521 is the product of three of these.)
+// The centre value is formed as A0 - (fm + fp), which equals 2/3 A0 - A2:
+// in FP32, fl(1/6) + fl(2/3) + fl(1/6) = 1 + 3.0e-8, so weighting A0 by the
+// rounded 2/3 would scale the zeroth moment (the density deviation P) by
+// 1 + 9e-8 per step over the three passes -- a mass drift eleven times that
+// of the rounded weights of the reference's own table.  Formed this way the
+// three values sum to A0 up to unbiased rounding.
 template <class R>
 __device__ __forceinline__ void recon3(R A0, R A1, R A2, R& fm, R& f0, R& fp) {
   const R t = A0 * (R)(1.0 / 6.0) + A2 * (R)0.5;
   fp = t + A1 * (R)0.5;
   fm = t - A1 * (R)0.5;
-  f0 = A0 * (R)(2.0 / 3.0) - A2;
+  f0 = A0 - (R)2 * t;
 }
 
 }  // namespace lbf

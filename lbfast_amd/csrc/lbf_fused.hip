@@ -516,18 +516,16 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   };
   // this lane's node sits at tile row w + 1, column pc
   const int pc = halo_wave ? (side ? XW + 2 : 1) : lane + 2;
-  auto phi_part = [&](int z, R& ctr) -> Part3<R> {
+  auto phi_part = [&](int z) -> Part3<R> {
     const int sl = z & 3;
     R v[3][3];
 #pragma unroll
     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
       for (int aa = 0; aa < 3; ++aa) v[bb][aa] = (R)sPH[sl][w + bb][pc - 1 + aa];
-    ctr = v[1][1];
     return part3(v);
   };
-  Part3<R> gpm{}, gp0{};  // partials of planes zsrc-1 and zsrc
-  R gc0 = 0;              // phi at the node in plane zsrc
+  Part3<R> gpm{}, gp0{};  // partials of planes zsrc-1 and zsrc (gp0.c: phi at the node)
   if constexpr (GI) {
     dma_phi(z0 - 2);
     dma_phi(z0 - 1);
@@ -535,9 +533,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
     dma_phi(z0 + 1);
     dma_wait_all();
     __syncthreads();
-    R c_unused;
-    gpm = phi_part(z0 - 2, c_unused);
-    gp0 = phi_part(z0 - 1, gc0);
+    gpm = phi_part(z0 - 2);
+    gp0 = phi_part(z0 - 1);
     __syncthreads();  // the slots of planes z0-2 and z0-1 are refilled in the first iterations
   }
 
@@ -569,10 +566,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       lds_raw<PREC, TWO, INLINE, GI>(cur, sPF[wave], lane, dp.dens, off.m, off.pair != 0);
       if constexpr (GI) {
         // the prep's fields 0-3 (prep_body_z, stored at storage precision)
-        R cp;
-        const Part3<R> gpp = phi_part(zsrc + 1, cp);
+        const Part3<R> gpp = phi_part(zsrc + 1);
         R gx, gy, gz, lap;
-        grad_lap(gpm, gp0, gpp, gc0, gx, gy, gz, lap);
+        grad_lap(gpm, gp0, gpp, gx, gy, gz, lap);
         const R mg = sqrt(gx * gx + gy * gy + gz * gz);
         const R im = rcp(mg + (R)1.0e-9);
         const R mgs = (R)(TA)mg;
@@ -580,10 +576,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         cur.g[1] = (TA)((R)(TA)(gy * im) * mgs);
         cur.g[2] = (TA)((R)(TA)(gz * im) * mgs);
         cur.g[3] = (TA)lap;
-        cur.phi = (TA)gc0;
+        cur.phi = (TA)gp0.c;
         gpm = gp0;
         gp0 = gpp;
-        gc0 = cp;
       }
     } else {
       if (obst) isf = (int8_t)isf_load(zsrc);

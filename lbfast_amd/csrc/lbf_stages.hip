@@ -317,7 +317,7 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
   const uint32_t pb = (uint32_t)(nxny * sizeof(TA));
   const TA* phi = (const TA*)a.phiS;
   TA* pre = (TA*)a.pre;
-  const R FOUR = 4, ONE = 1, G3 = (R)(3.0 / 216.0);
+  const R FOUR = 4, ONE = 1, HALF = 0.5, G3 = (R)(3.0 / 216.0);
   const R eps = (R)1.0e-9;
   constexpr bool store_g = !DO;
   auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
@@ -374,11 +374,10 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
     return v == 1 || v == -1;
   };
 
-  // per theta row h of this lane: phi partials of planes k-1 and k (pm, p0)
-  // and phi at the node in plane k (c0); per div row: theta partials of
+  // per theta row h of this lane: phi partials of planes k-1 and k (pm, p0;
+  // p0.c is phi at the node in plane k); per div row: theta partials of
   // planes k-3 and k-2 (em, e0)
   Part3<R> pm[2], p0[2], em[2], e0[2];
-  R c0[2];
   // outputs held for one iteration (see below)
   R hg[2][4], hd[2];
   size_t og[2] = {0, 0}, od[2] = {0, 0};
@@ -440,11 +439,16 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
           for (int aa = 0; aa < 3; ++aa) w[bb][aa] = sth[tk][2][t - 1 + bb][tj - 1 + aa];
-        ep.s = (w[0][0] + FOUR * w[0][1] + w[0][2]) + FOUR * (w[1][0] + FOUR * w[1][1] + w[1][2]) +
-               (w[2][0] + FOUR * w[2][1] + w[2][2]);
+        {
+          // smoothing of (theta_z - centre), as part3
+          const R ct = w[1][1];
+          ep.s = ((w[0][0] - ct) + FOUR * (w[0][1] - ct) + (w[0][2] - ct)) +
+                 FOUR * ((w[1][0] - ct) + (w[1][2] - ct)) + ((w[2][0] - ct) + FOUR * (w[2][1] - ct) + (w[2][2] - ct));
+          ep.c = ct;
+        }
         if (zo >= z0 && zo < zend) {
           const R d = G3 * ((em[h].dx + FOUR * e0[h].dx + ep.dx) + (em[h].dy + FOUR * e0[h].dy + ep.dy) +
-                            (ep.s - em[h].s));
+                            (ep.s - em[h].s)) + HALF * (ep.c - em[h].c);
           const int y = y0 - 1 + t;
           if (out_lane && y < ny && fluid(xo, y, zo)) {
             hd[h] = d;
@@ -475,11 +479,10 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
 #pragma unroll
           for (int aa = 0; aa < 3; ++aa) v[bb][aa] = (R)sphi[sp][t + bb][tj + aa];
         const Part3<R> pp = part3(v);
-        const R cp = v[1][1];
         if (have_theta) {
-          const R c = c0[h];
+          const R c = p0[h].c;
           R gx, gy, gz, lap;
-          grad_lap(pm[h], p0[h], pp, c, gx, gy, gz, lap);
+          grad_lap(pm[h], p0[h], pp, gx, gy, gz, lap);
           const int y = y0 - 1 + t;
           R thx = 0, thy = 0, thz = 0;
           if (fluid(xt, y, k)) {
@@ -506,7 +509,6 @@ __device__ __forceinline__ void prep_body_z(const StageArgs& a, int tz, int tile
         }
         pm[h] = p0[h];
         p0[h] = pp;
-        c0[h] = cp;
       }
     }
     // slot (k+3)&3 held plane k-1, last read in iteration k-2
