@@ -194,6 +194,18 @@ int lbf_end_step(lbf_ctx* ctx);
  * out[7] = fluid node count.  Synchronises. */
 int lbf_reduce_fp64(lbf_ctx* ctx, double out[8]);
 
+/* lbf_reduce_fp64 without allocation.  No counterpart in the reference: it
+ * is for callers that cannot call hipMalloc (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The partial sums
+ * go to a device array of the library (1024 records of 8 doubles plus the 8
+ * sums, 64 KiB + 64 B, held by each GPU the library runs on until the
+ * process ends) instead of the context's buffers; the grid-stride reduction
+ * covers every domain size lbf_create accepts within those 1024 records, so
+ * no size is refused.  The results equal lbf_reduce_fp64's bit for bit.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several host threads or contexts) race.  Synchronises. */
+int lbf_reduce_fp64_noalloc(lbf_ctx* ctx, double out[8]);
+
 int lbf_synchronize(lbf_ctx* ctx);
 int32_t lbf_get_step(const lbf_ctx* ctx);
 

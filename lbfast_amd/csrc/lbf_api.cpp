@@ -331,7 +331,7 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   }
   c->isfh = (int8_t*)A((size_t)(p.nx + 2) * (p.ny + 2) * (p.nz + 2));
   c->staging = A(c->N * 8);
-  c->nred = 1024;
+  c->nred = REDUCE_SCRATCH_BLOCKS;  // the no-alloc variant's partials match this launch
   c->partial = (double*)A((size_t)c->nred * 8 * sizeof(double));
   c->red = (double*)A(8 * sizeof(double));
   if (e != hipSuccess) {
@@ -767,6 +767,16 @@ int lbf_reduce_fp64(lbf_ctx* c, double out[8]) {
   if (!c || !out) return LBF_EINVAL;
   HIPCHK(c, launch_reduce(c->h[c->cur], c->phi[c->cur], c->isfh, c->dp, c->prec, c->partial, c->nred, c->red, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, c->red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBF_OK;
+}
+
+int lbf_reduce_fp64_noalloc(lbf_ctx* c, double out[8]) {
+  if (!c || !out) return LBF_EINVAL;
+  double* scr = nullptr;
+  HIPCHK(c, reduce_scratch_ptr(c->device, &scr));
+  HIPCHK(c, launch_reduce_scratch(c->h[c->cur], c->phi[c->cur], c->isfh, c->dp, c->prec, scr, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, scr + REDUCE_SCRATCH_SUMS, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBF_OK;
 }

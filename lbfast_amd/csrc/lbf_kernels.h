@@ -94,6 +94,16 @@ hipError_t launch_set_copy(void* dst, FieldLay dl, const void* src, FieldLay sl,
                            hipStream_t s);
 hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                          double* partial, int nblocks, double* out8, hipStream_t s);
+// Reductions into the library's own device array (no allocation): at most
+// REDUCE_SCRATCH_BLOCKS partial records of 8 doubles, then the 8 sums at
+// double offset REDUCE_SCRATCH_SUMS.  reduce_scratch_ptr gives the array's
+// device address on a device (looked up once per device).
+constexpr int REDUCE_SCRATCH_BLOCKS = 1024;
+constexpr size_t REDUCE_SCRATCH_SUMS = (size_t)REDUCE_SCRATCH_BLOCKS * 8;
+constexpr int REDUCE_SCRATCH_MAX_DEVICES = 64;
+hipError_t reduce_scratch_ptr(int device, double** out);
+hipError_t launch_reduce_scratch(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
+                                 double* scratch_dev, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
 // ctr[d] += 1 for d < n (the batch's per-domain step counters)
 hipError_t launch_tick(uint32_t* ctr, int n, hipStream_t s);

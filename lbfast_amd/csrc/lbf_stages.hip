@@ -14,6 +14,8 @@
 //                 and div(theta n) -- what the fused kernel's inline moments
 //                 and phase update consume.
 // The fused stage is the z-marching kernel of lbf_fused.hip.
+#include <mutex>
+
 #include "lbf_common.h"
 #include "lbf_kernels.h"
 #include "lbf_physics.h"
@@ -1206,6 +1208,54 @@ hipError_t launch_reduce(const void* S, const void* phi, const int8_t* isfh, con
   }
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, partial, nblocks, out8);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reductions that allocate nothing: the per-block partials and the eight sums
+// live in `scratch`, a file-scope device array that each GPU holds until the
+// process ends (64 KiB + 64 B).  Host code never passes the array's name to a
+// kernel or a copy: on the host it is only a stand-in for the symbol.  The
+// device address comes from hipGetSymbolAddress, once per device (with that
+// device current), and is kept per device: every GPU has its own copy.
+// Calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[REDUCE_SCRATCH_BLOCKS * 8 + 8];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[REDUCE_SCRATCH_MAX_DEVICES] = {};
+
+hipError_t reduce_scratch_ptr(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= REDUCE_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) {
+      const hipError_t e2 = hipSetDevice(cur);
+      if (e == hipSuccess) e = e2;
+    }
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (double*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+// The same kernels as launch_reduce, on min(REDUCE_SCRATCH_BLOCKS, ceil(nn /
+// BS)) blocks.  A node's (block, thread) and every partial sum are those of a
+// REDUCE_SCRATCH_BLOCKS-block launch (its extra blocks hold exact zeros), so
+// the sums equal lbf_reduce_fp64's bit for bit.  The grid-stride loop covers
+// any domain size, so the scratch bounds the block count, not the domain.
+hipError_t launch_reduce_scratch(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
+                                 double* scratch_dev, hipStream_t s) {
+  const size_t want = (dp.nn + BS - 1) / BS;
+  const int nb = (int)(want < (size_t)REDUCE_SCRATCH_BLOCKS ? (want ? want : 1) : (size_t)REDUCE_SCRATCH_BLOCKS);
+  return launch_reduce(S, phi, isfh, dp, prec, scratch_dev, nb, scratch_dev + REDUCE_SCRATCH_SUMS, s);
 }
 
 }  // namespace lbf

@@ -101,6 +101,11 @@ module lbfast_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(8)
     end function
+    integer(c_int) function lbf_reduce_fp64_noalloc(ctx, out) bind(C, name="lbf_reduce_fp64_noalloc")
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(8)
+    end function
     integer(c_int) function lbf_synchronize(ctx) bind(C, name="lbf_synchronize")
       import
       type(c_ptr), value :: ctx

@@ -25,7 +25,7 @@ EXPORTS = (
     "lbf_upload_field", "lbf_download_field", "lbf_upload_isfluid", "lbf_init_state",
     "lbf_step", "lbf_begin_step", "lbf_stage", "lbf_end_step", "lbf_reduce_fp64",
     "lbf_synchronize", "lbf_get_step", "lbf_field_device_ptr", "lbf_last_step_timing",
-    "lbf_set_tile", "lbf_set_timing",
+    "lbf_set_tile", "lbf_set_timing", "lbf_reduce_fp64_noalloc",
     "lbf_batch_create", "lbf_batch_destroy", "lbf_batch_last_error", "lbf_batch_step", "lbf_batch_capture",
     "lbf_batch_set_tile", "lbf_batch_autotune", "lbf_batch_synchronize", "lbf_batch_timing",
 )
@@ -63,6 +63,7 @@ def load(path: str = LIB_PATH):
     L.lbf_step.argtypes = [vp, ctypes.c_int32]
     L.lbf_stage.argtypes = [vp, ctypes.c_int]
     L.lbf_reduce_fp64.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.lbf_reduce_fp64_noalloc.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.lbf_get_step.argtypes = [vp]
     L.lbf_get_step.restype = ctypes.c_int32
     L.lbf_field_device_ptr.argtypes = [vp, ctypes.c_int]
@@ -202,6 +203,13 @@ class Domain:
     def reduce(self) -> np.ndarray:
         out = (ctypes.c_double * 8)()
         self._chk(self.L.lbf_reduce_fp64(self.h, out), "reduce")
+        return np.array(out[:])
+
+    def reduce_noalloc(self) -> np.ndarray:
+        """lbf_reduce_fp64_noalloc: the same sums through the library's own
+        device array (no allocation)."""
+        out = (ctypes.c_double * 8)()
+        self._chk(self.L.lbf_reduce_fp64_noalloc(self.h, out), "reduce_noalloc")
         return np.array(out[:])
 
     @property
