@@ -250,6 +250,8 @@ def ensemble(prec, ndom=96, steps=ENSEMBLE_STEPS, seed=7):
     looped = nodes * steps / (time.perf_counter() - t0) / 1e6
     b = hip.Batch(doms)
     ty, tz = b.autotune(4)
+    _, tzs, _, order = b.tiles()
+    classes = {int(L): tzs[i] for i, L in enumerate(Ls)}
     b.capture(2)
     b.step(2)
     b.synchronize()
@@ -261,8 +263,11 @@ def ensemble(prec, ndom=96, steps=ENSEMBLE_STEPS, seed=7):
     b.close()
     for d in doms:
         d.close()
+    kind = "as given" if order == sorted(order) else (
+        "largest first" if [int(Ls[d]) for d in order] == sorted([int(x) for x in Ls], reverse=True) else "alternating")
     return {"mlups_batched": round(batched, 1), "mlups_looped": round(looped, 1),
-            "speedup": round(batched / looped, 2), "tile": [ty, tz]}, nodes
+            "speedup": round(batched / looped, 2), "tile": [ty, tz],
+            "z_chunk_by_edge": {str(k): v for k, v in sorted(classes.items())}, "order": kind}, nodes
 
 
 def measured_traffic(workload):

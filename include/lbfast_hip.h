@@ -258,9 +258,19 @@ int lbf_batch_capture(lbf_batch* b, int32_t nsteps);
 /* Row tile (8 or 13) of the fused kernel and z chunks of the fused and
  * phase-field kernels, for every domain (chunks clipped to each nz). */
 int lbf_batch_set_tile(lbf_batch* b, int32_t tile_y, int32_t tile_z, int32_t tile_z_prep);
-/* Time trial_steps (even) steps of each candidate tile on the batch itself,
- * keep the fastest and restore every domain's state; reports the choice. */
+/* The table order: order[i] is the domain whose tiles come i-th in the
+ * grid of every launch (each XCD takes a contiguous eighth of it); a
+ * permutation of 0..ndomains-1.  Results do not depend on it. */
+int lbf_batch_set_order(lbf_batch* b, const int32_t* order);
+/* Time trial_steps (even) steps of each candidate on the batch itself, keep
+ * the fastest and restore every domain's state.  Searched in turn: the row
+ * tile with one z chunk for all domains (reported in tile_y, tile_z); the z
+ * chunk of each size class (domains of one nz); the table order (as given,
+ * largest first, largest and smallest alternating). */
 int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* tile_y, int32_t* tile_z);
+/* The batch's current choices: row tile, and per domain (arrays of
+ * ndomains) the fused and phase-field z chunks and the table order. */
+int lbf_batch_tiles(const lbf_batch* b, int32_t* tile_y, int32_t* tile_z, int32_t* tile_z_prep, int32_t* order);
 int lbf_batch_synchronize(lbf_batch* b);
 /* Elapsed milliseconds of the last lbf_batch_step call (HIP events). */
 int lbf_batch_timing(lbf_batch* b, double* ms);

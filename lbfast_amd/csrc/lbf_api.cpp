@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -797,6 +798,9 @@ struct lbf_batch {
   bool ho = false, two = false;
   int ty = 13;
   std::vector<int> tz, tz_prep;
+  // table position -> domain: the order in which the domains' tiles fill the
+  // grid (each XCD takes a contiguous eighth of it)
+  std::vector<int> order;
   FusedBatchEntry* ftab[2] = {};
   PrepBatchEntry* ptab[2] = {};
   int ftotal[2] = {}, ptotal[2] = {};
@@ -843,25 +847,26 @@ static int batch_tables(lbf_batch* b) {
     std::vector<FusedBatchEntry> f(nd);
     std::vector<PrepBatchEntry> q(nd);
     int ft = 0, pt = 0;
-    for (int d = 0; d < nd; ++d) {
+    for (int i = 0; i < nd; ++i) {
+      const int d = b->order[i];
       lbf_ctx* c = b->ctx[d];
       const int cur0 = c->cur, ucur0 = c->ucur;
       c->cur ^= par;
       c->ucur ^= par;
-      f[d].a = fused_args(c);
-      f[d].a.ty = b->ty;
-      f[d].a.tz = b->tz[d];
+      f[i].a = fused_args(c);
+      f[i].a.ty = b->ty;
+      f[i].a.tz = b->tz[d];
       // the seed of a step is the domain's step number, as in lbf_step:
       // 1 + the counter, which the step's tick advances afterwards
-      f[d].a.seed = 1u;
-      f[d].a.seedp = b->ctr + d;
-      f[d].tile0 = ft;
-      ft += fused_tile_count(f[d].a);
-      q[d].a = stage_args(c);
-      q[d].a.gf_from_pre = 1;
-      q[d].tz = b->tz_prep[d];
-      q[d].tile0 = pt;
-      if (b->two) pt += prep_tile_count(q[d].a, b->prec, b->tz_prep[d]);
+      f[i].a.seed = 1u;
+      f[i].a.seedp = b->ctr + d;
+      f[i].tile0 = ft;
+      ft += fused_tile_count(f[i].a);
+      q[i].a = stage_args(c);
+      q[i].a.gf_from_pre = 1;
+      q[i].tz = b->tz_prep[d];
+      q[i].tile0 = pt;
+      if (b->two) pt += prep_tile_count(q[i].a, b->prec, b->tz_prep[d]);
       c->cur = cur0;
       c->ucur = ucur0;
     }
@@ -948,6 +953,7 @@ int lbf_batch_create(lbf_ctx* const* ctxs, int32_t ndomains, lbf_batch** out) {
     b->tz.push_back(c->p.nz < 32 ? c->p.nz : 32);
     b->tz_prep.push_back(c->p.nz < 32 ? c->p.nz : 32);
   }
+  for (int d = 0; d < ndomains; ++d) b->order.push_back(d);
   bool ok = hipStreamCreate(&b->stream) == hipSuccess && hipEventCreate(&b->t0) == hipSuccess &&
             hipEventCreate(&b->t1) == hipSuccess && hipMalloc(&b->ctr, ndomains * sizeof(uint32_t)) == hipSuccess;
   for (lbf_ctx* c : b->ctx) {
@@ -1098,9 +1104,12 @@ int lbf_batch_timing(lbf_batch* b, double* ms) {
   return LBF_OK;
 }
 
-// The project's autotuner: time `trial_steps` (even) steps of each candidate
-// tile (row tile 8 or 13; z chunks 16..128 for both kernels) on the batch
-// itself, keep the fastest, and restore every domain's state afterwards.
+// The project's autotuner: times `trial_steps` (even) steps of each candidate
+// on the batch itself, keeps the fastest, and restores every domain's state
+// afterwards.  Candidates, searched in turn: the row tile (8 or 13) with one z
+// chunk (16..128) for every domain; then the z chunk of each size class
+// (domains of one nz) with the others held; then the table order (as given,
+// largest first, or largest and smallest alternating).
 int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int32_t* best_tz) {
   if (!b) return LBF_EINVAL;
   if (trial_steps < 2 || (trial_steps & 1)) return bfail(b, LBF_EINVAL, "trial_steps must be even and >= 2");
@@ -1127,23 +1136,95 @@ int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int3
       snap.push_back(q);
     }
   }
+  // one timed trial of the batch's current configuration (warm-up pair, then
+  // trial_steps steps; even counts keep the buffer parities)
+  auto trial = [&](double* ms) -> int {
+    int rc = batch_tables(b);
+    for (int s = 0; s < 2 && !rc; ++s) rc = batch_enqueue(b, s & 1);
+    if (rc) return rc;
+    BHIPCHK(b, hipEventRecord(b->t0, b->stream));
+    for (int s = 0; s < trial_steps && !rc; ++s) rc = batch_enqueue(b, s & 1);
+    if (rc) return rc;
+    BHIPCHK(b, hipEventRecord(b->t1, b->stream));
+    BHIPCHK(b, hipEventSynchronize(b->t1));
+    float m = 0;
+    BHIPCHK(b, hipEventElapsedTime(&m, b->t0, b->t1));
+    *ms = m;
+    return LBF_OK;
+  };
+  const int nd = (int)b->ctx.size();
+  auto set_uniform = [&](int tz) {
+    for (int d = 0; d < nd; ++d) {
+      const int nz = b->ctx[d]->p.nz;
+      b->tz[d] = b->tz_prep[d] = tz < nz ? tz : nz;
+    }
+  };
+  batch_drop_graph(b);
+  // 1. row tile and one z chunk for every domain
   const int tys[2] = {8, 13}, tzs[4] = {16, 32, 64, 128};
   double best = 1e300;
   int bty = b->ty, btz = b->tz.empty() ? 32 : b->tz[0];
   for (int ty : tys)
     for (int tz : tzs) {
-      int rc = lbf_batch_set_tile(b, ty, tz, tz);
+      b->ty = ty;
+      set_uniform(tz);
+      double m = 0;
+      int rc = trial(&m);
       if (rc) { release(); return rc; }
-      for (int s = 0; s < 2 && !rc; ++s) rc = batch_enqueue(b, s & 1);  // warm-up pair
-      BHIPCHK(b, hipEventRecord(b->t0, b->stream));
-      for (int s = 0; s < trial_steps && !rc; ++s) rc = batch_enqueue(b, s & 1);
-      BHIPCHK(b, hipEventRecord(b->t1, b->stream));
-      if (rc) { release(); return rc; }
-      BHIPCHK(b, hipEventSynchronize(b->t1));
-      float m = 0;
-      BHIPCHK(b, hipEventElapsedTime(&m, b->t0, b->t1));
       if (m < best) { best = m; bty = ty; btz = tz; }
     }
+  b->ty = bty;
+  set_uniform(btz);
+  // 2. the z chunk of each size class (domains of one nz), the others held:
+  // short domains lose least to the chunks' two halo planes with one chunk,
+  // tall ones may balance better with more
+  std::vector<int> classes;
+  for (lbf_ctx* c : b->ctx)
+    if (std::find(classes.begin(), classes.end(), c->p.nz) == classes.end()) classes.push_back(c->p.nz);
+  if (classes.size() > 1) {
+    for (int nz : classes) {
+      const int cand[5] = {16, 32, 64, 128, nz};
+      int keep = -1;
+      for (int tz : cand) {
+        if (tz > nz) continue;
+        std::vector<int> save = b->tz;
+        for (int d = 0; d < nd; ++d)
+          if (b->ctx[d]->p.nz == nz) b->tz[d] = b->tz_prep[d] = tz;
+        double m = 0;
+        int rc = trial(&m);
+        if (rc) { release(); return rc; }
+        if (m < best * 0.995) { best = m; keep = tz; }
+        b->tz = save;
+        b->tz_prep = save;
+      }
+      if (keep > 0)
+        for (int d = 0; d < nd; ++d)
+          if (b->ctx[d]->p.nz == nz) b->tz[d] = b->tz_prep[d] = keep;
+    }
+  }
+  // 3. table order: as given, largest first (the tallest work items start
+  // first), or largest and smallest alternating (each XCD's contiguous eighth
+  // of the grid holds a mix)
+  {
+    std::vector<int> by(nd);
+    for (int d = 0; d < nd; ++d) by[d] = d;
+    std::stable_sort(by.begin(), by.end(), [&](int x, int y) { return b->ctx[x]->N > b->ctx[y]->N; });
+    std::vector<int> alt;
+    for (int lo = 0, hi = nd - 1; lo <= hi;) {
+      alt.push_back(by[lo++]);
+      if (lo <= hi) alt.push_back(by[hi--]);
+    }
+    const std::vector<int> given = b->order;
+    std::vector<int> bestord = given;
+    for (const std::vector<int>* o : {&by, &alt}) {
+      b->order = *o;
+      double m = 0;
+      int rc = trial(&m);
+      if (rc) { b->order = given; release(); return rc; }
+      if (m < best * 0.995) { best = m; bestord = *o; }
+    }
+    b->order = bestord;
+  }
   // restore the snapshot (trial steps were even: the parities are unchanged)
   size_t k = 0;
   for (lbf_ctx* c : b->ctx) {
@@ -1154,11 +1235,41 @@ int lbf_batch_autotune(lbf_batch* b, int32_t trial_steps, int32_t* best_ty, int3
   }
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   release();
-  int rc = lbf_batch_set_tile(b, bty, btz, btz);
+  int rc = batch_tables(b);
   if (rc) return rc;
-  if (best_ty) *best_ty = bty;
+  if (best_ty) *best_ty = b->ty;
   if (best_tz) *best_tz = btz;
   b->timing_valid = false;
+  return LBF_OK;
+}
+
+int lbf_batch_set_order(lbf_batch* b, const int32_t* order) {
+  if (!b || !order) return LBF_EINVAL;
+  const int nd = (int)b->ctx.size();
+  std::vector<int> o(order, order + nd), seen(nd, 0);
+  for (int v : o) {
+    if (v < 0 || v >= nd || seen[v]) return bfail(b, LBF_EINVAL, "order must be a permutation of 0..%d", nd - 1);
+    seen[v] = 1;
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  {
+    int rc = batch_refresh(b);
+    if (rc) return rc;
+  }
+  batch_drop_graph(b);
+  b->order = o;
+  return batch_tables(b);
+}
+
+int lbf_batch_tiles(const lbf_batch* b, int32_t* tile_y, int32_t* tile_z, int32_t* tile_z_prep, int32_t* order) {
+  if (!b) return LBF_EINVAL;
+  const int nd = (int)b->ctx.size();
+  if (tile_y) *tile_y = b->ty;
+  for (int d = 0; d < nd; ++d) {
+    if (tile_z) tile_z[d] = b->tz[d];
+    if (tile_z_prep) tile_z_prep[d] = b->tz_prep[d];
+    if (order) order[d] = b->order[d];
+  }
   return LBF_OK;
 }
 

@@ -601,10 +601,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         m.xx += m.u * m.u; m.yy += m.v * m.v; m.zz += m.w * m.w;
         m.xy += m.u * m.v; m.xz += m.u * m.w; m.yz += m.v * m.w;
       }
-      R phl = 0, rinv = (R)1, Fx, Fy, Fz;
+      R phl = 0, Fx, Fy, Fz;
       const bool solid = obst && isf == 0;
       if constexpr (TWO) phl = (R)cur.phi;
-      if (TWO && dp.dens) rinv = rcp(ph.rho_r * phl + ((R)1 - phl) * ph.rho_b);
+      const NodeRates<R> nr = node_rates<TWO, R>(dp, phl);
+      const R rinv = nr.irho;
       if constexpr (!INLINE) {
         Fx = (R)cur.g[0] * rinv;
         Fy = (R)cur.g[1] * rinv;
@@ -624,7 +625,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
           }
         }
         R tFx, tFy, tFz;
-        moments_stage<TWO, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+        moments_stage<TWO, R>(dp, nr, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
         // the reference stores S and the forces at storage precision between
         // the moments and fused kernels
         m.u = C::round(m.u); m.v = C::round(m.v); m.w = C::round(m.w);
@@ -660,7 +661,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
             const R cssq = (R)(1.0 / 3.0);
             const R pxx = m.xx - m.u * m.u, pyy = m.yy - m.v * m.v, pzz = m.zz - m.w * m.w;
             const R pxy = m.xy - m.u * m.v, pxz = m.xz - m.u * m.w, pyz = m.yz - m.v * m.w;
-            const R om = relax_rate<TWO, R>(dp, phl, rinv, pxx, pyy, pzz, pxy, pxz, pyz);
+            const R om = relax_rate<TWO, R>(dp, nr, pxx, pyy, pzz, pxy, pxz, pyz);
 #pragma unroll
             for (int q = 0; q < 10; ++q) lbc[q] = 0;
 #pragma unroll
@@ -689,7 +690,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       in.xx = m.xx - m.u * m.u; in.yy = m.yy - m.v * m.v; in.zz = m.zz - m.w * m.w;
       in.xy = m.xy - m.u * m.v; in.xz = m.xz - m.u * m.w; in.yz = m.yz - m.v * m.w;
       in.ax = Fx; in.ay = Fy; in.az = Fz;
-      in.om = relax_rate<TWO, R>(dp, phl, rinv, in.xx, in.yy, in.zz, in.xy, in.xz, in.yz);
+      in.om = relax_rate<TWO, R>(dp, nr, in.xx, in.yy, in.zz, in.xy, in.xz, in.yz);
       if constexpr (PF) {
         // every store of this iteration is issued: stream the next plane in
         lds_wait_all();

@@ -25,15 +25,42 @@ __device__ __forceinline__ int8_t isf_at(const int8_t* isfh, int x, int y, int z
 }
 
 // ---------------------------------------------------------------------------
+// Per-node rates shared by the moments stage, the collision and the near-wall
+// closure: 1/rho_phi (two-component DENSRATIO; 1 otherwise), the kinematic
+// viscosity (the density-weighted two-component mix, or visc1) and the plain
+// relaxation rate 1/(3 nu + 1/2).  Computed once per node: the quarter-rate
+// reciprocals are not repeated in the branches that use them.
+template <class R>
+struct NodeRates {
+  R irho, visc, om0;
+};
+template <bool TWO, class R>
+__device__ __forceinline__ NodeRates<R> node_rates(const DevParams& dp, R phl) {
+  const Phys<R>& ph = dp.ph<R>();
+  NodeRates<R> r;
+  r.irho = (TWO && dp.dens) ? rcp(ph.rho_r * phl + ((R)1 - phl) * ph.rho_b) : (R)1;
+  if (TWO) {
+    // without DENSRATIO the reference's viscosity mix still reads rho_r/rho_b
+    r.visc = (ph.rho_r * ph.visc1 * phl + ((R)1 - phl) * ph.visc2 * ph.rho_b) * r.irho;
+    r.om0 = rcp(r.visc * (R)3 + (R)0.5);
+  } else {
+    r.visc = ph.visc1;
+    r.om0 = ph.omega;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
 // Moments stage of one node (lb_cuda_moments_module.f90:104-339), in place on
 // the loaded moments m; returns the stored force tF.  Expression order
 // follows the reference; divisions by rho_phi, the INTERFACE_INCOMP
-// denominator and tau*cs^2 are multiplications by one reciprocal each.
+// denominator and tau*cs^2 are multiplications by one reciprocal each, the
+// first and last taken from the node's rates (nr = node_rates(dp, phl)).
 // rflux: the REPULSIVE_FLUX flux of this node (locaux 6..8), or null.
 template <bool TWO, class R>
-__device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx, R gfy, R gfz, R lap, R dvt,
-                                              R uox, R uoy, R uoz, Mom<R>& m, R& tFx, R& tFy, R& tFz,
-                                              const R* rflux = nullptr) {
+__device__ __forceinline__ void moments_stage(const DevParams& dp, const NodeRates<R>& nr, R phl, R gfx, R gfy,
+                                              R gfz, R lap, R dvt, R uox, R uoy, R uoz, Mom<R>& m, R& tFx, R& tFy,
+                                              R& tFz, const R* rflux = nullptr) {
   const Phys<R>& ph = dp.ph<R>();
   const R cssq = (R)(1.0 / 3.0), invcssq = (R)3, HALF = (R)0.5, ONE = (R)1;
   const bool dens = TWO && dp.dens != 0;
@@ -43,7 +70,7 @@ __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx,
   if (TWO) mask = (R)4.0 * phl * (ONE - phl);
   if (dens) {
     rhophi = rr * phl + (ONE - phl) * rb;
-    irho = rcp(rhophi);
+    irho = nr.irho;
     if (dp.buoy) {
       Fx = (rhophi - (rr + rb) * HALF) * ph.fx;
       Fy = (rhophi - (rr + rb) * HALF) * ph.fy;
@@ -80,9 +107,9 @@ __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx,
   if (dens) {
     const R pxx = m.xx - uox * uox, pyy = m.yy - uoy * uoy, pzz = m.zz - uoz * uoz;
     const R pxy = m.xy - uox * uoy, pxz = m.xz - uox * uoz, pyz = m.yz - uoy * uoz;
-    const R visc = (rr * ph.visc1 * phl + ((R)1.0 - phl) * ph.visc2 * rb) * irho;
-    const R tau = (visc * invcssq + HALF);
-    vt = visc * rcp(tau * cssq);
+    // visc / (tau cs^2) with tau = 3 visc + 1/2, as 3 visc / tau: 1/tau is
+    // the node's plain relaxation rate
+    vt = (nr.visc * invcssq) * nr.om0;
     Fx = Fx - mask * vt * (pxx * grx + pxy * gry + pxz * grz);
     Fy = Fy - mask * vt * (pyy * gry + pxy * grx + pyz * grz);
     Fz = Fz - mask * vt * (pzz * grz + pxz * grx + pyz * gry);
@@ -126,18 +153,15 @@ __device__ __forceinline__ void moments_stage(const DevParams& dp, R phl, R gfx,
 // Q = Pneq:Pneq, C = 0.053*18.0 evaluated in single precision as the
 // reference's default-real literals are.
 template <bool TWO, class R>
-__device__ __forceinline__ R relax_rate(const DevParams& dp, R phl, R irho, R pxx, R pyy, R pzz, R pxy, R pxz,
-                                        R pyz) {
-  const Phys<R>& ph = dp.ph<R>();
-  const R visc = TWO ? (ph.rho_r * ph.visc1 * phl + ((R)1 - phl) * ph.visc2 * ph.rho_b) * irho : ph.visc1;
+__device__ __forceinline__ R relax_rate(const DevParams& dp, const NodeRates<R>& nr, R pxx, R pyy, R pzz, R pxy,
+                                        R pxz, R pyz) {
   if (dp.smag) {
     const R QQ = pxx * pxx + pyy * pyy + pzz * pzz + (R)2 * (pxy * pxy + pxz * pxz + pyz * pyz);
     const R C = (R)(0.053f * 18.0f);
-    const R t3 = (R)3 * visc;
-    return rcp((R)0.5 + (R)(1.0 / 6.0) * ((R)3 * visc + sqrt(t3 * t3 + C * sqrt((R)2 * QQ) * irho)));
+    const R t3 = (R)3 * nr.visc;
+    return rcp((R)0.5 + (R)(1.0 / 6.0) * ((R)3 * nr.visc + sqrt(t3 * t3 + C * sqrt((R)2 * QQ) * nr.irho)));
   }
-  if (!TWO) return ph.omega;
-  return rcp(visc * (R)3 + (R)0.5);
+  return nr.om0;
 }
 
 }  // namespace lbf

@@ -664,10 +664,11 @@ __global__ void __launch_bounds__(BS) k_moments(StageArgs a) {
       const TA* rf = (const TA*)a.rep + sbase(6, x, y, z, sx, ny) + 3 * sx;
       rfl[0] = (R)rf[0]; rfl[1] = (R)rf[sx]; rfl[2] = (R)rf[2 * sx];
     }
-    moments_stage<true, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz,
-                           dp.rep ? rfl : nullptr);
+    moments_stage<true, R>(dp, node_rates<true, R>(dp, phl), phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx,
+                           tFy, tFz, dp.rep ? rfl : nullptr);
   } else {
-    moments_stage<false, R>(dp, phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx, tFy, tFz);
+    moments_stage<false, R>(dp, node_rates<false, R>(dp, phl), phl, gfx, gfy, gfz, lap, dvt, uox, uoy, uoz, m, tFx,
+                            tFy, tFz);
   }
   store_mom<PREC>((T*)a.S + n10, sx, zero, m, dp);
   TA* fr = (TA*)a.frc;

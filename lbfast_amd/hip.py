@@ -28,6 +28,7 @@ EXPORTS = (
     "lbf_set_tile", "lbf_set_timing", "lbf_reduce_fp64_noalloc",
     "lbf_batch_create", "lbf_batch_destroy", "lbf_batch_last_error", "lbf_batch_step", "lbf_batch_capture",
     "lbf_batch_set_tile", "lbf_batch_autotune", "lbf_batch_synchronize", "lbf_batch_timing",
+    "lbf_batch_set_order", "lbf_batch_tiles",
 )
 
 
@@ -80,6 +81,8 @@ def load(path: str = LIB_PATH):
     L.lbf_batch_capture.argtypes = [vp, i32]
     L.lbf_batch_set_tile.argtypes = [vp, i32, i32, i32]
     L.lbf_batch_autotune.argtypes = [vp, i32, pi32, pi32]
+    L.lbf_batch_set_order.argtypes = [vp, pi32]
+    L.lbf_batch_tiles.argtypes = [vp, pi32, pi32, pi32, pi32]
     L.lbf_batch_synchronize.argtypes = [vp]
     L.lbf_batch_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -269,6 +272,18 @@ class Batch:
         self._chk(self.L.lbf_batch_autotune(self.h, trial_steps, ctypes.byref(ty), ctypes.byref(tz)),
                   "lbf_batch_autotune")
         return ty.value, tz.value
+
+    def set_order(self, order):
+        arr = (ctypes.c_int32 * len(self.domains))(*[int(x) for x in order])
+        self._chk(self.L.lbf_batch_set_order(self.h, arr), "lbf_batch_set_order")
+
+    def tiles(self):
+        """(row tile, per-domain fused z chunks, per-domain prep z chunks, table order)."""
+        n = len(self.domains)
+        ty = ctypes.c_int32()
+        tz, tzp, order = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
+        self._chk(self.L.lbf_batch_tiles(self.h, ctypes.byref(ty), tz, tzp, order), "lbf_batch_tiles")
+        return ty.value, list(tz), list(tzp), list(order)
 
     def synchronize(self):
         self._chk(self.L.lbf_batch_synchronize(self.h), "lbf_batch_synchronize")

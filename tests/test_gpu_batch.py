@@ -145,6 +145,10 @@ def test_batch_autotune_restores_state():
         if tune:
             ty, tz = b.autotune(2)
             assert ty in (8, 13) and tz in (16, 32, 64, 128)
+            ty2, tzs, tzps, order = b.tiles()
+            assert ty2 == ty and sorted(order) == list(range(len(ds)))
+            for (p, _), z in zip(doms, tzs):
+                assert 1 <= z <= p.nz
         b.step(4)
         b.synchronize()
         results.append([_state(d, p) for (p, _), d in zip(doms, ds)])
@@ -236,6 +240,39 @@ def test_mixed_size_batch_against_oracle():
                               np.stack([o.get_field(i) for i in FIELD_UOLD]))}
         print(f"[batch]   {p.nx}x{p.ny}x{p.nz}: {err}", flush=True)
         assert max(err.values()) < 1e-5, err
+    b.close()
+    for d in ds:
+        d.close()
+
+
+def test_batch_order_and_class_tiles_leave_results_unchanged():
+    # The table order and per-domain z chunks change only which block
+    # computes which node: every domain still equals its single-domain run.
+    print("[batch] order and per-domain chunks", flush=True)
+    doms = _domains(SPECS, PREC_FP32)
+    want = []
+    for p, f in doms:
+        d = hip.Domain(p)
+        d.load(f)
+        d.step(5)
+        want.append(_state(d, p))
+        d.close()
+    ds = []
+    for p, f in doms:
+        d = hip.Domain(p)
+        d.load(f)
+        ds.append(d)
+    b = hip.Batch(ds)
+    b.set_order([2, 0, 3, 1])
+    assert b.tiles()[3] == [2, 0, 3, 1]
+    b.step(2)
+    b.set_order([3, 2, 1, 0])
+    b.step(3)
+    b.synchronize()
+    for (p, _), d, w in zip(doms, ds, want):
+        assert np.array_equal(_state(d, p), w)
+    with pytest.raises(hip.LbfError):
+        b.set_order([0, 0, 1, 2])
     b.close()
     for d in ds:
         d.close()
