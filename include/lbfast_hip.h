@@ -26,6 +26,8 @@
  *   PHI_int_boundary_cuda           (driver:1965-2009) -> lbf_stage(LBF_STAGE_PHI_INT_BOUNDARY)
  *   thinfilm_scan_mark_cuda         (driver:850-880)   -> lbf_stage(LBF_STAGE_THINFILM_SCAN)
  *   repulsive_flux_normal_cuda      (driver:882-910)   -> lbf_stage(LBF_STAGE_REPULSIVE_FLUX)
+ *   phi_sum_count_cuda + apply_lagrangian_phi_cuda (driver:2011-2077, BCPHIFLUX)
+ *                                                      -> lbf_stage(LBF_STAGE_PHI_LAGRANGE)
  *   fused_LB_cuda under BOUNCE_BACK (driver:1464-1465, fused_LB_kernel_bb)
  *                                                      -> lbf_stage(LBF_STAGE_FUSED), bounce_back = 1
  *   exchange_*_intpbc (mpi_module.f90:2203-3524)       -> in-kernel periodic wrap (no call)
@@ -109,6 +111,8 @@ extern "C" {
 #define LBF_STAGE_LB_INT_BOUNDARY   7  /* LB_int_boundary_kernel           */
 #define LBF_STAGE_THINFILM_SCAN     8  /* thinfilm_scan_mark_kernel (REPULSIVE_FLUX) */
 #define LBF_STAGE_REPULSIVE_FLUX    9  /* repulsive_flux_normal_kernel     */
+#define LBF_STAGE_PHI_LAGRANGE     10  /* phi_sum_count_kernel + apply_lagrangian_phi_kernel
+                                          (BCPHIFLUX), after PHI_INT_BOUNDARY */
 
 /* Parameters: the &simulation / &fluid namelists (LBFAST.f90:32-60) that
  * the time step reads, the defines.h switches, and the derived quantities
@@ -127,7 +131,8 @@ typedef struct lbf_params {
   int32_t bounce_back;           /* BOUNCE_BACK (with internal_obstacles)    */
   int32_t smagorinsky;           /* SMAGORINSKI                             */
   int32_t repulsive_flux;        /* REPULSIVE_FLUX (two-component)          */
-  int32_t reserved_i;            /* zero                                    */
+  int32_t bcphiflux;             /* BCPHIFLUX: global phase-mass correction
+                                    (two-component with internal obstacles) */
   double fx, fy, fz;             /* body force                              */
   double visc1, visc2;           /* kinematic viscosities                   */
   double rho_r, rho_b;           /* phase densities (DENSRATIO)             */

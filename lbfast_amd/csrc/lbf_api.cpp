@@ -45,6 +45,7 @@ struct lbf_ctx {
   void* staging = nullptr;
   double* partial = nullptr;
   double* red = nullptr;
+  double* lag = nullptr;  // BCPHIFLUX: initial phi sum, last sum, count, correction
   int nred = 0;
   int cur = 0, ucur = 0;
   int step = 0;
@@ -268,6 +269,10 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   if (p.densratio && !p.twocomponent) return fail(nullptr, LBF_EINVAL, "DENSRATIO requires TWOCOMPONENT");
   if (p.precision < LBF_PREC_FP64 || p.precision > LBF_PREC_BF16S) return fail(nullptr, LBF_EINVAL, "bad precision");
   if (p.repulsive_flux && !p.twocomponent) return fail(nullptr, LBF_EINVAL, "REPULSIVE_FLUX requires TWOCOMPONENT");
+  // bcs_mesoscopic_phifields, which holds the correction, is compiled only
+  // with INTERNAL_OBSTACLES and TWOCOMPONENT (boundary_cds_module.f90:1345-1351)
+  if (p.bcphiflux && !(p.twocomponent && p.internal_obstacles))
+    return fail(nullptr, LBF_EINVAL, "BCPHIFLUX requires TWOCOMPONENT and INTERNAL_OBSTACLES");
   if (p.repulsive_flux && (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S))
     return fail(nullptr, LBF_EINVAL, "REPULSIVE_FLUX runs the stage path: FP64, FP32 or MIXED storage");
   if (p.visc1 <= 0 || (p.twocomponent && p.visc2 <= 0)) return fail(nullptr, LBF_EINVAL, "viscosity must be > 0");
@@ -335,6 +340,7 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
   c->nred = REDUCE_SCRATCH_BLOCKS;  // the no-alloc variant's partials match this launch
   c->partial = (double*)A((size_t)c->nred * 8 * sizeof(double));
   c->red = (double*)A(8 * sizeof(double));
+  if (p.bcphiflux) c->lag = (double*)A(4 * sizeof(double));
   if (e != hipSuccess) {
     std::string m = std::string("device allocation failed: ") + hipGetErrorString(e);
     lbf_destroy(c);
@@ -593,10 +599,14 @@ int lbf_init_state(lbf_ctx* c) {
     HIPCHK(c, launch_fill_zero(c->rep, 6 * c->NS * c->asz, c->stream));
     HIPCHK(c, launch_fill_zero(c->rep_mask, c->N, c->stream));
   }
-  // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0
+  // bcs_phi before the loop (integrator_module.f90:179-182), with aux = 0;
+  // under BCPHIFLUX the initial phase sum is taken first (initial conditions,
+  // init_conditions_module.f90:469-490) and the correction follows the ghost update
   if (c->p.twocomponent && c->p.internal_obstacles) {
     StageArgs a = stage_args(c);
+    if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, true, c->stream));
     HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+    if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->step = 0;
@@ -632,6 +642,7 @@ static int staged_step(lbf_ctx* c) {
   if (two) {
     HIPCHK(c, launch_stage(LBF_STAGE_NORM, a, c->prec, c->stream));
     if (obst) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+    if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
     HIPCHK(c, launch_stage(LBF_STAGE_DIV_THETA_N, a, c->prec, c->stream));
     if (c->dp.rep) {
       HIPCHK(c, launch_stage(LBF_STAGE_THINFILM_SCAN, a, c->prec, c->stream));
@@ -670,6 +681,7 @@ int lbf_step(lbf_ctx* c, int32_t nsteps) {
       a.gf_from_pre = 1;
       HIPCHK(c, launch_phase_prep(a, c->prec, c->tz_prep, c->stream));
       if (c->p.internal_obstacles) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+      if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
     }
     int rc = run_fused(c, true);
     if (rc) return rc;
@@ -739,6 +751,13 @@ int lbf_stage(lbf_ctx* c, int id) {
     case LBF_STAGE_REPULSIVE_FLUX:
       if (!c->dp.rep) return LBF_OK;
       break;
+    case LBF_STAGE_PHI_LAGRANGE: {
+      if (!c->lag) return LBF_OK;
+      StageArgs a = stage_args(c);
+      HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      return LBF_OK;
+    }
     case LBF_STAGE_FUSED: {
       if (c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S)
         return fail(c, LBF_EINVAL, "the stage-path fused kernel supports FP64/FP32/MIXED storage");

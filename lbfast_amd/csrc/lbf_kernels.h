@@ -104,6 +104,10 @@ constexpr int REDUCE_SCRATCH_MAX_DEVICES = 64;
 hipError_t reduce_scratch_ptr(int device, double** out);
 hipError_t launch_reduce_scratch(const void* S, const void* phi, const int8_t* isfh, const DevParams& dp, int prec,
                                  double* scratch_dev, hipStream_t s);
+// BCPHIFLUX phase-mass correction: init = sum only (into lag[0]); otherwise
+// sum, count, correction and its application to phi S
+hipError_t launch_phi_lagrange(const StageArgs& a, int prec, double* partial, int nblocks, double* lag, bool init,
+                               hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
 // ctr[d] += 1 for d < n (the batch's per-domain step counters)
 hipError_t launch_tick(uint32_t* ctr, int n, hipStream_t s);

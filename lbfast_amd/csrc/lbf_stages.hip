@@ -1147,6 +1147,102 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 }
 
 // ---------------------------------------------------------------------------
+// BCPHIFLUX: the global phase-mass correction of phi_sum_count_kernel and
+// apply_lagrangian_phi_kernel (lb_cuda_boundary_module.f90:461-566), driven as
+// boundary_cds_module.f90:1864-1884 drives them: the sum of phi over the
+// |isfluid| == 1 nodes and the count of those with 0.5 < phi < 0.9, then
+// corr = (sum_ini - sum) / count added to the counted nodes.  The sum is
+// taken in a fixed order (block partials, then one thread), the correction
+// is formed on the device, so no host round trip sits in the step.
+// lag[0] = sum at initialisation, lag[1] = last sum, lag[2] = last count,
+// lag[3] = last correction.
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_phi_lag_sum(StageArgs a, double* partial) {
+  typedef typename Aux<PREC>::T TA;
+  __shared__ double sh[2][BS];
+  const DevParams& dp = a.dp;
+  double sum = 0, cnt = 0;
+  for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < dp.nn; t += (size_t)gridDim.x * BS) {
+    int x, y, z;
+    node_of(t, dp, x, y, z);
+    const int8_t isf = isf_at(a.isfh, x, y, z, dp.nx, dp.ny, dp.nz);
+    if (isf != 1 && isf != -1) continue;
+    const double ph = (double)((const TA*)a.phiS)[t];
+    sum += ph;
+    if (ph > 0.5 && ph < 0.9) cnt += 1.0;
+  }
+  sh[0][threadIdx.x] = sum;
+  sh[1][threadIdx.x] = cnt;
+  __syncthreads();
+  for (int st = BS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) partial[(size_t)blockIdx.x * 2 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_phi_lag_final(const double* partial, int nb, double* lag, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double sum = 0, cnt = 0;
+  for (int b = 0; b < nb; ++b) {
+    sum += partial[2 * b];
+    cnt += partial[2 * b + 1];
+  }
+  if (init) {
+    lag[0] = sum;  // global_phi_sum_ini (init_conditions_module.f90:469-490)
+    lag[1] = sum;
+    lag[2] = cnt;
+    lag[3] = 0.0;
+    return;
+  }
+  lag[1] = sum;
+  lag[2] = cnt;
+  // corr = dphi / real(global_count): the count is converted to a default
+  // (single precision) real, as the reference's real() without a kind does.
+  // No counted node (an empty interface band) gives no correction.
+  lag[3] = cnt > 0 ? (lag[0] - sum) / (double)(float)cnt : 0.0;
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_phi_lag_apply(StageArgs a, const double* lag) {
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  int x, y, z;
+  const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (!node_of(t, dp, x, y, z)) return;
+  const int8_t isf = isf_at(a.isfh, x, y, z, dp.nx, dp.ny, dp.nz);
+  if (isf != 1 && isf != -1) return;
+  TA* phi = (TA*)a.phiS;
+  const double ph = (double)phi[t];
+  if (ph > 0.5 && ph < 0.9) phi[t] = (TA)(ph + lag[3]);
+}
+
+template <int PREC>
+static hipError_t lag_t(const StageArgs& a, double* partial, int nb, double* lag, bool init, hipStream_t s) {
+  hipLaunchKernelGGL(k_phi_lag_sum<PREC>, dim3(nb), dim3(BS), 0, s, a, partial);
+  hipLaunchKernelGGL(k_phi_lag_final, dim3(1), dim3(64), 0, s, (const double*)partial, nb, lag, init ? 1 : 0);
+  if (!init)
+    hipLaunchKernelGGL(k_phi_lag_apply<PREC>, dim3((unsigned)((a.dp.nn + BS - 1) / BS)), dim3(BS), 0, s, a,
+                       (const double*)lag);
+  return hipGetLastError();
+}
+
+hipError_t launch_phi_lagrange(const StageArgs& a, int prec, double* partial, int nblocks, double* lag, bool init,
+                               hipStream_t s) {
+  switch (prec) {
+    case PREC_FP64: return lag_t<PREC_FP64>(a, partial, nblocks, lag, init, s);
+    case PREC_FP32: return lag_t<PREC_FP32>(a, partial, nblocks, lag, init, s);
+    case PREC_MIXED: return lag_t<PREC_MIXED>(a, partial, nblocks, lag, init, s);
+    case PREC_FP16S: return lag_t<PREC_FP16S>(a, partial, nblocks, lag, init, s);
+    case PREC_BF16S: return lag_t<PREC_BF16S>(a, partial, nblocks, lag, init, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 reductions over |isfluid| == 1 nodes: per-block partials, then one
 // block sums the partials in a fixed order (deterministic).
 template <int PREC>

@@ -3,7 +3,8 @@
  * states.  Select with -D on the amdflang line, e.g.
  *   -DLAPLACE  -DLATTICE=27 -DHIGHORDER  -DPRC=4
  * LATTICE (19|27), HIGHORDER, TWOCOMPONENT, DENSRATIO, INTERNAL_OBSTACLES,
- * BUOYANCY_FORCING, BOUNCE_BACK, SMAGORINSKI, REPULSIVE_FLUX; precision:
+ * BUOYANCY_FORCING, BOUNCE_BACK, SMAGORINSKI, REPULSIVE_FLUX, BCPHIFLUX (the
+ * global phase-mass correction; two-component runs with obstacles); precision:
  * PRC (8|4), MIXEDPRC (PRC 8 arithmetic, 4-byte storage), STORE_FP16 /
  * STORE_BF16 (16-bit scaled-deviation storage, FP32 arithmetic); test case:
  * TAYLORGREEN, LAPLACE, LAMBTEST, POISEUILLE, TWOPOISEUILLE (default: the

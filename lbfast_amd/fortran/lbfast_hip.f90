@@ -22,7 +22,7 @@ module lbfast_hip
   type, bind(C) :: lbf_params
     integer(c_int32_t) :: nx, ny, nz, lattice, highorder, twocomponent, densratio, interface_incomp
     integer(c_int32_t) :: internal_obstacles, buoyancy_forcing, pbc(3), precision
-    integer(c_int32_t) :: bounce_back, smagorinsky, repulsive_flux, reserved_i
+    integer(c_int32_t) :: bounce_back, smagorinsky, repulsive_flux, bcphiflux
     real(c_double) :: fx, fy, fz, visc1, visc2, rho_r, rho_b, tau_diff, sigma, width, wettab_r
     real(c_double) :: tau1, omega, tau2, beta, kapp, sharp_c, store_scale(10)
     real(c_double) :: a_rep, q_th, cos_opp_t, reserved_d

@@ -33,11 +33,13 @@ HFIELD_NAMES = ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
 
 STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N = 1, 2, 3
 STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY = 4, 5, 6, 7
-STAGE_THINFILM_SCAN, STAGE_REPULSIVE_FLUX = 8, 9
+STAGE_THINFILM_SCAN, STAGE_REPULSIVE_FLUX, STAGE_PHI_LAGRANGE = 8, 9, 10
 # integrator_module.f90:278-524 (one FLIP half step)
-# integrator_module.f90:266-545; the repulsive stages and LB_int_boundary are
-# no-ops unless REPULSIVE_FLUX / INTERNAL_OBSTACLES without BOUNCE_BACK
-STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N, STAGE_THINFILM_SCAN,
+# integrator_module.f90:266-545; the repulsive stages, the phase-mass
+# correction and LB_int_boundary are no-ops unless REPULSIVE_FLUX / BCPHIFLUX /
+# INTERNAL_OBSTACLES without BOUNCE_BACK (bcs_mesoscopic_phifields runs
+# PHI_int_boundary, then the BCPHIFLUX correction: boundary_cds_module.f90:1347-1887)
+STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_PHI_LAGRANGE, STAGE_DIV_THETA_N, STAGE_THINFILM_SCAN,
               STAGE_REPULSIVE_FLUX, STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY)
 
 
@@ -49,7 +51,7 @@ class LbfParams(ctypes.Structure):
         ("interface_incomp", ctypes.c_int32), ("internal_obstacles", ctypes.c_int32),
         ("buoyancy_forcing", ctypes.c_int32), ("pbc", ctypes.c_int32 * 3),
         ("precision", ctypes.c_int32), ("bounce_back", ctypes.c_int32),
-        ("smagorinsky", ctypes.c_int32), ("repulsive_flux", ctypes.c_int32), ("reserved_i", ctypes.c_int32),
+        ("smagorinsky", ctypes.c_int32), ("repulsive_flux", ctypes.c_int32), ("bcphiflux", ctypes.c_int32),
         ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
         ("visc1", ctypes.c_double), ("visc2", ctypes.c_double),
         ("rho_r", ctypes.c_double), ("rho_b", ctypes.c_double),
@@ -99,7 +101,7 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
                 fx=0.0, fy=0.0, fz=0.0, visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=1.0,
                 tau_diff=0.1, sigma=0.0, width=4.0, wettab_r=90.0,
                 interface_incomp=None, buoyancy_forcing=False, bounce_back=False, smagorinsky=False,
-                repulsive_flux=False, a_rep=0.0, q_th=0.125, cos_opp_t=-0.8) -> LbfParams:
+                repulsive_flux=False, a_rep=0.0, q_th=0.125, cos_opp_t=-0.8, bcphiflux=False) -> LbfParams:
     """Build a parameter block. ``interface_incomp`` defaults to ``densratio``
     because defines.h:72-74 defines INTERFACE_INCOMP whenever DENSRATIO is set."""
     p = LbfParams()
@@ -125,6 +127,7 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
     p.smagorinsky = int(bool(smagorinsky))
     p.repulsive_flux = int(bool(repulsive_flux) and p.twocomponent)
     p.a_rep, p.q_th, p.cos_opp_t = a_rep, q_th, cos_opp_t
+    p.bcphiflux = int(bool(bcphiflux))
     return derive(p)
 
 

@@ -210,7 +210,8 @@ struct Sim : SimBase {
   std::vector<R> pop;  // post-collision populations of the last fused stage
   int cur = 0;         // index of the S buffer
   int stepno = 0;
-  bool two, dens, incomp, obst, ho, bb, smag, rep;
+  bool two, dens, incomp, obst, ho, bb, smag, rep, lagr;
+  double phi_sum_ini = 0;  // BCPHIFLUX: global_phi_sum_ini
   R Minv[27][27];
 
   explicit Sim(const lbf_params& pp) : p(pp) {
@@ -236,6 +237,7 @@ struct Sim : SimBase {
     bb = obst && p.bounce_back != 0;
     smag = p.smagorinsky != 0;
     rep = two && p.repulsive_flux != 0;
+    lagr = two && obst && p.bcphiflux != 0;
     for (int k = 0; k < 27; ++k)
       for (int l = 0; l < 27; ++l) Minv[k][l] = (R)minv_d3q27(k, l);
   }
@@ -841,6 +843,45 @@ struct Sim : SimBase {
         }
   }
 
+  // ---- BCPHIFLUX phase-mass correction ------------------------------------------
+  // phi_sum_count_kernel / apply_lagrangian_phi_kernel (lb_cuda_boundary_module.f90:
+  // 461-566) as boundary_cds_module.f90:1864-1884 drives them, without open
+  // boundaries (global_phi_change = 0): the phi sum over |isfluid| == 1 nodes
+  // (double), the count of those with 0.5 < phi < 0.9, corr = (sum_ini - sum) /
+  // real(count) with the count as a default (single precision) real, added to
+  // the counted nodes at storage precision.  The reference refreshes no phi
+  // halo afterwards, so its fused stage would collide periodic halo images
+  // with the uncorrected value; here the halo is refreshed, as an in-kernel
+  // periodic wrap reads the corrected value.
+  void phi_sums(const std::vector<T>& ph, double& sum, long& cnt) const {
+    sum = 0;
+    cnt = 0;
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const size_t n = I(i, j, k);
+          if (std::abs((int)isf[n]) != 1) continue;
+          const double v = (double)ph[n];
+          sum += v;
+          if (v > 0.5 && v < 0.9) ++cnt;
+        }
+  }
+  void phi_lagrange(std::vector<T>& ph) {
+    double sum;
+    long cnt;
+    phi_sums(ph, sum, cnt);
+    const double corr = cnt > 0 ? (phi_sum_ini - sum) / (double)(float)cnt : 0.0;
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const size_t n = I(i, j, k);
+          if (std::abs((int)isf[n]) != 1) continue;
+          const double v = (double)ph[n];
+          if (v > 0.5 && v < 0.9) ph[n] = (T)(v + corr);
+        }
+    halo(ph);
+  }
+
   // ---- driver-level entry points ---------------------------------------------------
   int begin_step() override { ++stepno; return 0; }
   int end_step() override { cur = 1 - cur; return 0; }
@@ -860,6 +901,7 @@ struct Sim : SimBase {
       case LBF_STAGE_LB_INT_BOUNDARY: if (!obst || bb) return 0; lb_int_boundary(S, O, pS); return 0;
       case LBF_STAGE_THINFILM_SCAN: if (!rep) return 0; thinfilm_scan(pS); return 0;
       case LBF_STAGE_REPULSIVE_FLUX: if (!rep) return 0; repulsive_flux(pS); return 0;
+      case LBF_STAGE_PHI_LAGRANGE: if (!lagr) return 0; phi_lagrange(pS); return 0;
       default: err = "unknown stage"; return LBF_EINVAL;
     }
   }
@@ -870,6 +912,7 @@ struct Sim : SimBase {
       begin_step();
       stage(LBF_STAGE_NORM);
       stage(LBF_STAGE_PHI_INT_BOUNDARY);
+      stage(LBF_STAGE_PHI_LAGRANGE);
       stage(LBF_STAGE_DIV_THETA_N);
       stage(LBF_STAGE_THINFILM_SCAN);
       stage(LBF_STAGE_REPULSIVE_FLUX);
@@ -887,7 +930,12 @@ struct Sim : SimBase {
     for (int f = 0; f < 10; ++f) h[1 - cur][f] = h[cur][f];
     phi[1 - cur] = phi[cur];
     for (auto& a : aux) std::fill(a.begin(), a.end(), T(0));
+    if (lagr) {  // global_phi_sum_ini (init_conditions_module.f90:469-490)
+      long cnt;
+      phi_sums(phi[cur], phi_sum_ini, cnt);
+    }
     if (two && obst) phi_int_boundary(phi[cur]);  // bcs_phi before the loop
+    if (lagr) phi_lagrange(phi[cur]);
     if (two) halo(phi[cur]);
     for (int f = 0; f < 10; ++f) halo(h[cur][f]);
     // the copy O := S happened before the halos were filled; fill O's too so

@@ -10,8 +10,8 @@ from lbfast_amd.params import (make_params, PREC_FP64, PREC_FP32, PREC_MIXED, HF
 
 def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
     """(params, fields, raw isfluid or None) for a named test workload.
-    Suffixes: ``_bb`` BOUNCE_BACK, ``_smag`` SMAGORINSKI."""
-    for suffix, key in (("_bb", "bounce_back"), ("_smag", "smagorinsky")):
+    Suffixes: ``_bb`` BOUNCE_BACK, ``_smag`` SMAGORINSKI, ``_bcphi`` BCPHIFLUX."""
+    for suffix, key in (("_bb", "bounce_back"), ("_smag", "smagorinsky"), ("_bcphi", "bcphiflux")):
         if name.endswith(suffix):
             p, f, raw = case(name[: -len(suffix)], n, precision, seed)
             setattr(p, key, 1)

@@ -139,6 +139,15 @@ def test_smagorinsky(name, prec, path):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("name", ["chan27h_bcphi", "chan19_bcphi"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+@pytest.mark.parametrize("path", ["fast", "staged"])
+def test_bcphiflux(name, prec, path):
+    """BCPHIFLUX: the global phase-mass correction after the wetting update."""
+    res, bad = run_pair(name, prec, (40, 24, 32), 8, path=path)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_repulsive_flux(prec):
     """REPULSIVE_FLUX: facing-partner scan, repulsive flux and its force (the

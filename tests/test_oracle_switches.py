@@ -13,6 +13,11 @@ REPULSIVE_FLUX.
   scan pairs each interfacial node with its mirror node, the flux is the
   closed form A q w(q) along the normal, antisymmetric across the film; a
   thick film has no facing partner within the 7^3 window.
+* BCPHIFLUX (lb_cuda_boundary_module.f90:461-566, boundary_cds_module.f90:
+  1864-1884): the correction stage equals a brute-force restatement (sum of
+  phi over |isfluid| = 1 nodes, count of 0.5 < phi < 0.9, shift of the
+  counted nodes by (initial sum - sum) / count); over a run the phase mass
+  stays at its initial value to round-off, where without it it drifts.
 """
 import numpy as np
 import pytest
@@ -172,3 +177,62 @@ def test_repulsive_force_reaches_the_moments():
     d = np.abs(runs[1] - runs[0])
     assert d.max() > 1e-6
     assert d[:, :, :8].max() < 1e-14 and d[:, :, 16:].max() < 1e-14  # only near the film
+
+
+def _bcphi_case(n=(24, 20, 16), bcphi=True, seed=5):
+    p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, internal_obstacles=True,
+                    pbc=(1, 0, 1), visc1=0.05, visc2=0.1, rho_r=2.0, rho_b=1.0, tau_diff=0.1, sigma=0.005,
+                    width=4.0, fx=1e-5, precision=PREC_FP64, bcphiflux=bcphi)
+    raw = setups.random_spheres(n, fill=0.15, rmin=2.0, rmax=3.0, seed=seed, margin=2)
+    isf = oracle.classify_isfluid(p, raw)
+    f = setups.two_layer_channel(n, raw, frac=0.4, width=4.0, rho_r=2.0, rho_b=1.0, axis=1)
+    f["phi"] = setups.solid_phi_average(f["phi"], isf, 27)
+    return p, f, isf
+
+
+def test_bcphiflux_stage_equals_brute_force():
+    p, f, isf = _bcphi_case()
+    o = oracle.Oracle(p)
+    o.load(f, isf)
+    fl = np.abs(isf) == 1
+    ini = o.get_field(FIELD_PHI)[fl].sum()  # after the initial correction the sum is the initial one
+    o.step(3)
+    o.begin_step()
+    o.stage(1)  # compute_norm
+    o.stage(2)  # PHI_int_boundary
+    before = o.get_field(FIELD_PHI)
+    o.stage(10)  # the correction
+    after = o.get_field(FIELD_PHI)
+    band = fl & (before > 0.5) & (before < 0.9)
+    assert band.sum() > 10
+    corr = (ini - before[fl].sum()) / float(np.float32(band.sum()))
+    want = np.where(band, before + corr, before)
+    assert abs(corr) > 1e-12  # the phase update does drift between corrections
+    # the oracle sums ~8k values in sequence, numpy pairwise: the sums differ
+    # by ~1e-11, the corrections by ~1e-14
+    assert np.abs(after - want).max() < 1e-12
+    assert abs(after[fl].sum() - ini) < 1e-12 * ini
+
+
+def test_bcphiflux_keeps_phase_mass():
+    drift = {}
+    for bcphi in (True, False):
+        p, f, isf = _bcphi_case(bcphi=bcphi)
+        o = oracle.Oracle(p)
+        o.load(f, isf)
+        fl = np.abs(isf) == 1
+        m0 = o.get_field(FIELD_PHI)[fl].sum()
+        worst = 0.0
+        for _ in range(30):
+            # the step's stage order (STEP_ORDER), sampling the phase mass
+            # right after the correction stage
+            o.begin_step()
+            for sid in (1, 2, 10):
+                o.stage(sid)
+            worst = max(worst, abs(o.get_field(FIELD_PHI)[fl].sum() - m0) / m0)
+            for sid in (3, 8, 9, 4, 5, 6, 7):
+                o.stage(sid)
+            o.end_step()
+        drift[bcphi] = worst
+    assert drift[True] < 1e-13
+    assert drift[False] > 1e3 * max(drift[True], 1e-16)
