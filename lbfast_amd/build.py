@@ -35,6 +35,10 @@ FORTRAN_VARIANTS = {
     "laplace27h_fp64": ["-DLAPLACE", "-DLATTICE=27", "-DHIGHORDER", "-DPRC=8"],
     "laplace27h_fp16": ["-DLAPLACE", "-DLATTICE=27", "-DHIGHORDER", "-DSTORE_FP16"],
     "pois19_bb_fp64": ["-DPOISEUILLE", "-DLATTICE=19", "-DBOUNCE_BACK", "-DPRC=8"],
+    "pois19_fp64": ["-DPOISEUILLE", "-DLATTICE=19", "-DPRC=8", "-DDOXDMF"],
+    "twopois27h_fp64": ["-DTWOPOISEUILLE", "-DLATTICE=27", "-DHIGHORDER", "-DPRC=8", "-DDOXDMF", "-DWRITEPRESS"],
+    "lamb27h_fp32": ["-DLAMBTEST", "-DLATTICE=27", "-DHIGHORDER", "-DPRC=4", "-DPRINTPHI"],
+    "twopois27h_bcphi_fp64": ["-DTWOPOISEUILLE", "-DLATTICE=27", "-DHIGHORDER", "-DPRC=8", "-DBCPHIFLUX"],
 }
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

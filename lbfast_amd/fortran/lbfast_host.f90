@@ -11,9 +11,15 @@
 !  * the time loop in FLIP/FLOP pairs (integrator_module.f90:266-827), with
 !    the statistics.f90 diagnostics every stamp_term steps: Taylor-Green
 !    kinetic energy against exp(-6 nu k^2 t), the Laplace pressure jump
-!    sigma_eff = (p_in - p_out) R / 2, global mass;
-!  * performance as LBFAST.f90:500 (MLUPS), RAW output (lraw) and the restart
-!    file (lwriterestart / lrestart) in the reference's formats.
+!    sigma_eff = (p_in - p_out) R / 2, the Lamb oscillation record (lamb.dat),
+!    global mass;
+!  * the end-of-run benchmark comparisons (statistics.f90:688-1114): the
+!    one- and two-fluid Poiseuille profiles against the analytic ones, the
+!    Lamb oscillation period against the viscous-corrected theory;
+!  * performance as LBFAST.f90:500 (MLUPS); frames every stamp steps as RAW
+!    (lraw, with XDMF sidecars under DOXDMF) and VTK image data (lvtk), 2-D
+!    plane frames every stamp2D steps (nplanes, ndir, npoint), and the restart
+!    file (lwriterestart / lrestart), in the reference's formats.
 #include "lbfast_defines.h"
 program lbfast_host
   use iso_c_binding
@@ -143,7 +149,7 @@ program lbfast_host
 #ifdef BUOYANCY_FORCING
   p%buoyancy_forcing = 1
 #endif
-  p%bounce_back = 0; p%smagorinsky = 0; p%repulsive_flux = 0
+  p%bounce_back = 0; p%smagorinsky = 0; p%repulsive_flux = 0; p%bcphiflux = 0
 #ifdef BOUNCE_BACK
   p%bounce_back = 1
 #endif
@@ -152,6 +158,9 @@ program lbfast_host
 #endif
 #ifdef REPULSIVE_FLUX
   p%repulsive_flux = 1
+#endif
+#ifdef BCPHIFLUX
+  p%bcphiflux = 1
 #endif
   p%pbc = [pbc_x, pbc_y, pbc_z]
 #if defined(STORE_FP16)
@@ -177,6 +186,7 @@ program lbfast_host
   write (6, '(a,7i2,i4)') 'lbfast_host: switches densratio incomp obstacles buoyancy bounce_back smagorinsky'// &
     ' repulsive pbc', p%densratio, p%interface_incomp, p%internal_obstacles, p%buoyancy_forcing, &
     p%bounce_back, p%smagorinsky, p%repulsive_flux, p%pbc(1) * 100 + p%pbc(2) * 10 + p%pbc(3)
+  write (6, '(a,i2)') 'lbfast_host: bcphiflux', p%bcphiflux
   write (6, '(a,5g16.8)') 'lbfast_host: tau1 omega tau2 beta kapp', p%tau1, p%omega, p%tau2, p%beta, p%kapp
   flush (6)
 
@@ -230,13 +240,18 @@ program lbfast_host
   deallocate (fld)
 
   if (lprint .and. lraw) call write_isfluid_frame()
-  if (lprint .and. lraw .and. .not. lrestart) call write_frame(iframe)
+  if (lprint .and. (lraw .or. lvtk) .and. .not. lrestart) call write_frame(iframe)
+  if (lprint .and. nplanes > 0 .and. .not. lrestart) call write_frame_2d(iframe2d)
 
   ! ---- time loop (integrator_module.f90:266-827) ------------------------------------------
   ! The reference advances FLIP/FLOP pairs: 2*ceil(nsteps/2) steps.  Steps
   ! are enqueued in chunks that end on the next step at which output,
   ! statistics or the wall-clock check is due.
   e0 = kinetic_energy()
+#if defined(LAMBTEST) && defined(TWOCOMPONENT)
+  ! open_lamb (statistics.f90:277-383, called at integrator_module.f90:204)
+  open (unit=142, file='lamb.dat', action='write', status='replace')
+#endif
   call report(0)
   ntot = 2 * ((max(nsteps, 0) + 1) / 2)
   stop_early = .false.
@@ -245,6 +260,7 @@ program lbfast_host
   do while (step < ntot)
     nxt = ntot
     if (stamp > 0 .and. lprint) nxt = min(nxt, (step / stamp + 1) * stamp)
+    if (stamp2D > 0 .and. lprint .and. nplanes > 0) nxt = min(nxt, (step / stamp2D + 1) * stamp2D)
     if (stamp_term > 0) nxt = min(nxt, (step / stamp_term + 1) * stamp_term)
     if (time_limit > 0 .and. every_time_check > 0) nxt = min(nxt, (step / every_time_check + 1) * every_time_check)
     chunk = nxt - step
@@ -254,7 +270,13 @@ program lbfast_host
     if (lprint .and. stamp > 0) then
       if (mod(step, stamp) == 0) then
         iframe = iframe + 1
-        if (lraw) call write_frame(iframe)
+        if (lraw .or. lvtk) call write_frame(iframe)
+      end if
+    end if
+    if (lprint .and. stamp2D > 0 .and. nplanes > 0) then
+      if (mod(step, stamp2D) == 0) then
+        iframe2d = iframe2d + 1
+        call write_frame_2d(iframe2d)
       end if
     end if
     if (stamp_term > 0) then
@@ -282,6 +304,14 @@ program lbfast_host
   mlups = real(lx, db) * real(ly, db) * real(lz, db) * real(step, db) / max(tend, 1.0e-9_db) / 1.0e6_db
   write (6, '(a,i10,a,f12.4,a,f14.2)') 'lbfast_host: steps', step, '  seconds', tend, '  MLUPS', mlups
   if (lwriterestart) call write_restart(iframe, iframe2d)
+  ! compare_benchmark (statistics.f90:688-1114, called at LBFAST.f90:530)
+#if defined(LAMBTEST) && defined(TWOCOMPONENT)
+  close (142)
+  call lamb_period()
+#endif
+#if defined(POISEUILLE) || (defined(TWOPOISEUILLE) && defined(TWOCOMPONENT))
+  call poiseuille_profile()
+#endif
   call lbf_destroy(ctx)
   write (6, '(a)') 'lbfast_host: done'
 
@@ -338,7 +368,8 @@ contains
 #endif
 #endif
     ! phi is not reset per node: a solid node keeps the value of the last
-    ! fluid node before it, as the reference's loop leaves it
+    ! fluid node before it, as the reference's loop leaves it, until
+    ! solid_phi_average replaces it in runs with obstacles
     do gk = 1, lz
       do gj = 1, ly
         do gi = 1, lx
@@ -389,6 +420,45 @@ contains
           f(gi, gj, gk, 5) = u * u; f(gi, gj, gk, 6) = v * v; f(gi, gj, gk, 7) = w * w
           f(gi, gj, gk, 8) = u * v; f(gi, gj, gk, 9) = u * w; f(gi, gj, gk, 10) = v * w
           f(gi, gj, gk, 11) = phi
+        end do
+      end do
+    end do
+#if defined(TWOCOMPONENT) && defined(INTERNAL_OBSTACLES)
+    call solid_phi_average(f(:, :, :, 11))
+#endif
+  end subroutine
+
+  ! init_conditions_module.f90:468-542: phi at each solid node (isfluid 0)
+  ! becomes the mean of its in-range non-solid lattice neighbours, weighted
+  ! 1, 1/sqrt(2), 1/sqrt(3) by link length; a solid node with no such
+  ! neighbour gets 0 (the reference divides 0 by 0 there)
+  subroutine solid_phi_average(ph)
+    real(db), intent(inout) :: ph(:, :, :)
+    real(db) :: num, den, s
+    integer :: gi, gj, gk, a, b, c, ni, nj, nk, d2
+    ! only solid nodes are written and only non-solid ones read: in place
+    do gk = 1, lz
+      do gj = 1, ly
+        do gi = 1, lx
+          if (isf(gi, gj, gk) /= 0) cycle
+          num = 0; den = 0
+          do c = -1, 1
+            do b = -1, 1
+              do a = -1, 1
+                d2 = a * a + b * b + c * c
+                if (d2 == 0 .or. (LATTICE == 19 .and. d2 == 3)) cycle
+                ni = gi + a; nj = gj + b; nk = gk + c
+                if (ni < 1 .or. ni > lx .or. nj < 1 .or. nj > ly .or. nk < 1 .or. nk > lz) cycle
+                if (isf(ni, nj, nk) == 0) cycle
+                s = 1.0_db
+                if (d2 == 2) s = 1.0_db / sqrt(2.0_db)
+                if (d2 == 3) s = 1.0_db / sqrt(3.0_db)
+                num = num + s * ph(ni, nj, nk); den = den + s
+              end do
+            end do
+          end do
+          ph(gi, gj, gk) = 0
+          if (den > 0) ph(gi, gj, gk) = num / den
         end do
       end do
     end do
@@ -455,6 +525,9 @@ contains
     p_out = rho_out * cssq * buf2(ic, jc, kc)
     write (6, '(a,i10,a,es20.12,a,es20.12,a,es20.12,a,es20.12)') 'LAPLACE step', st, ' sigma_eff', &
       (p_in - p_out) * rad * 0.5_db, ' delta_p', p_in - p_out, ' radius', rad, ' phase_mass', red(5)
+#if defined(LAMBTEST)
+    call lamb_line(st)
+#endif
 #else
     write (6, '(a,i10,a,es20.12)') 'MASS step', st, ' sum_P', red(1)
 #endif
@@ -497,17 +570,13 @@ contains
     write (fmtnumb, '(i6.6)') max(0, min(999999, n))
   end function
 
-  ! print_raw_sync (print_module.f90:2007-2180): output/out_rho_NNNNNN.raw,
-  ! out_vel_NNNNNN.raw (u, v, w interleaved per node) and, with WRITEPRESS
-  ! in a two-component build, out_press_NNNNNN.raw; real(4), stream access.
-  ! rho is rho_r phi + (1 - phi) rho_b with DENSRATIO (phi with PRINTPHI),
-  ! phi in a two-component build without it, P in a one-component build.
-  subroutine write_frame(fr)
-    integer, intent(in) :: fr
-    real(4), allocatable :: o(:, :, :), vel(:, :, :, :)
-    integer :: nxs, nys, nzs, c
-    nxs = lx / stepskip; nys = ly / stepskip; nzs = lz / stepskip
-    allocate (o(nxs, nys, nzs), vel(3, nxs, nys, nzs))
+  ! copy_print (print_module.f90:638-776): the print observables at every
+  ! stepskip-th node: rho is rho_r phi + (1 - phi) rho_b with DENSRATIO (phi
+  ! with PRINTPHI), phi in a two-component build without it, P in a
+  ! one-component build; vel holds u, v, w interleaved per node; press is P
+  subroutine copy_print(rp, vp, pp)
+    real(4), intent(out) :: rp(:, :, :), vp(:, :, :, :), pp(:, :, :)
+    integer :: c
 #if defined(TWOCOMPONENT)
     call download(LBF_FIELD_PHI, buf)
 #if defined(DENSRATIO) && !defined(PRINTPHI)
@@ -516,22 +585,188 @@ contains
 #else
     call download(LBF_FIELD_P, buf)
 #endif
-    call skip_copy(buf, o)
-    call write_raw4('output/out_rho_'//fmtnumb(fr)//'.raw', o)
+    call skip_copy(buf, rp)
     do c = 1, 3
       call download(int(LBF_FIELD_U + c - 1, c_int), buf)
-      call skip_copy(buf, o)
-      vel(c, :, :, :) = o
+      call skip_copy(buf, pp)
+      vp(c, :, :, :) = pp
     end do
-    open (unit=345, file='output/out_vel_'//fmtnumb(fr)//'.raw', status='replace', action='write', &
-          access='stream', form='unformatted')
-    write (345) vel
-    close (345)
-#if defined(TWOCOMPONENT) && defined(WRITEPRESS)
     call download(LBF_FIELD_P, buf)
-    call skip_copy(buf, o)
-    call write_raw4('output/out_press_'//fmtnumb(fr)//'.raw', o)
+    call skip_copy(buf, pp)
+  end subroutine
+
+  ! print_raw_sync (print_module.f90:2007-2180) and print_vtk_sync: frames
+  ! output/out_rho_NNNNNN, out_vel_NNNNNN and, with WRITEPRESS in a two-
+  ! component build, out_press_NNNNNN; real(4), stream access.  lraw writes
+  ! .raw files (with an .xdmf description beside each under DOXDMF), lvtk
+  ! VTK image data (.vti, raw appended data)
+  subroutine write_frame(fr)
+    integer, intent(in) :: fr
+    real(4), allocatable :: rp(:, :, :), vp(:, :, :, :), pp(:, :, :)
+    integer :: nxs, nys, nzs
+    nxs = lx / stepskip; nys = ly / stepskip; nzs = lz / stepskip
+    allocate (rp(nxs, nys, nzs), vp(3, nxs, nys, nzs), pp(nxs, nys, nzs))
+    call copy_print(rp, vp, pp)
+    if (lraw) then
+      call write_raw4('output/out_rho_'//fmtnumb(fr)//'.raw', rp)
+      call write_raw4v('output/out_vel_'//fmtnumb(fr)//'.raw', vp)
+#if defined(DOXDMF)
+      call write_xdmf('output/out_rho_'//fmtnumb(fr), 'rho', [nxs, nys, nzs], 1)
+      call write_xdmf('output/out_vel_'//fmtnumb(fr), 'vel', [nxs, nys, nzs], 3)
 #endif
+#if defined(TWOCOMPONENT) && defined(WRITEPRESS)
+      call write_raw4('output/out_press_'//fmtnumb(fr)//'.raw', pp)
+#if defined(DOXDMF)
+      call write_xdmf('output/out_press_'//fmtnumb(fr), 'press', [nxs, nys, nzs], 1)
+#endif
+#endif
+    end if
+    if (lvtk) then
+      call write_vti('output/out_rho_'//fmtnumb(fr)//'.vti', 'rho', rp, [nxs, nys, nzs], 1)
+      call write_vti('output/out_vel_'//fmtnumb(fr)//'.vti', 'vel', vp, [nxs, nys, nzs], 3)
+#if defined(TWOCOMPONENT) && defined(WRITEPRESS)
+      call write_vti('output/out_press_'//fmtnumb(fr)//'.vti', 'press', pp, [nxs, nys, nzs], 1)
+#endif
+    end if
+  end subroutine
+
+  ! driver_print_raw_sync2D / print_raw_sync2D (print_module.f90:2307-2660):
+  ! for each plane l, the slice normal to axis ndir(l) at node npoint(l) of
+  ! the print observables, as output/out_<ndir>_<npoint>_rho_NNNNNN.raw,
+  ! ..._vel_... (and ..._press_... with WRITEPRESS), with .xdmf sidecars under
+  ! DOXDMF
+  subroutine write_frame_2d(fr)
+    integer, intent(in) :: fr
+    real(4), allocatable :: rp(:, :, :), vp(:, :, :, :), pp(:, :, :), s1(:, :, :), s3(:, :, :, :), s2(:, :, :)
+    integer :: nxs, nys, nzs, l, pt, d(3)
+    character(len=64) :: stem
+    nxs = lx / stepskip; nys = ly / stepskip; nzs = lz / stepskip
+    allocate (rp(nxs, nys, nzs), vp(3, nxs, nys, nzs), pp(nxs, nys, nzs))
+    call copy_print(rp, vp, pp)
+    do l = 1, nplanes
+      pt = npoint(l) / stepskip
+      select case (ndir(l))
+      case (1)
+        d = [1, nys, nzs]
+        if (pt < 1 .or. pt > nxs) call plane_error(l)
+        s1 = rp(pt:pt, :, :); s2 = pp(pt:pt, :, :); s3 = vp(:, pt:pt, :, :)
+      case (2)
+        d = [nxs, 1, nzs]
+        if (pt < 1 .or. pt > nys) call plane_error(l)
+        s1 = rp(:, pt:pt, :); s2 = pp(:, pt:pt, :); s3 = vp(:, :, pt:pt, :)
+      case (3)
+        d = [nxs, nys, 1]
+        if (pt < 1 .or. pt > nzs) call plane_error(l)
+        s1 = rp(:, :, pt:pt); s2 = pp(:, :, pt:pt); s3 = vp(:, :, :, pt:pt)
+      case default
+        write (6, '(a)') 'wrong argument in ndir for 2d plane print: only 1 to 3 is permitted'
+        stop 2
+      end select
+      write (stem, '(a,i0,a,i0,a)') 'output/out_', ndir(l), '_', npoint(l), '_'
+      call write_raw4(trim(stem)//'rho_'//fmtnumb(fr)//'.raw', s1)
+      call write_raw4v(trim(stem)//'vel_'//fmtnumb(fr)//'.raw', s3)
+#if defined(DOXDMF)
+      call write_xdmf(trim(stem)//'rho_'//fmtnumb(fr), 'rho', d, 1)
+      call write_xdmf(trim(stem)//'vel_'//fmtnumb(fr), 'vel', d, 3)
+#endif
+#if defined(TWOCOMPONENT) && defined(WRITEPRESS)
+      call write_raw4(trim(stem)//'press_'//fmtnumb(fr)//'.raw', s2)
+#if defined(DOXDMF)
+      call write_xdmf(trim(stem)//'press_'//fmtnumb(fr), 'press', d, 1)
+#endif
+#endif
+    end do
+  end subroutine
+
+  subroutine plane_error(l)
+    integer, intent(in) :: l
+    write (6, '(a,i0,a,i0,a,i0)') 'lbfast_host: 2-D plane ', l, ' at node ', npoint(l), &
+      ' lies outside the domain along axis ', ndir(l)
+    stop 2
+  end subroutine
+
+  ! XDMF description of a RAW file (print_module.f90:2020-2064): a uniform
+  ! 3DCoRectMesh, origin 0, spacing 1, dimensions slowest axis first, one
+  ! node-centred attribute (a vector when ncomp = 3) in little-endian float32
+  subroutine write_xdmf(stem, name, d, ncomp)
+    character(len=*), intent(in) :: stem, name
+    integer, intent(in) :: d(3), ncomp
+    character(len=64) :: dims
+    integer :: u, k
+    write (dims, '(i0,a,i0,a,i0)') d(3), ' ', d(2), ' ', d(1)
+    k = index(stem, '/', back=.true.)
+    open (newunit=u, file=stem//'.xdmf', status='replace', action='write')
+    write (u, '(a)') '<?xml version="1.0" ?>'
+    write (u, '(a)') '<Xdmf Version="3.0">'
+    write (u, '(a)') '  <Domain>'
+    write (u, '(a)') '    <!-- Uniform Grid -->'
+    if (ncomp == 3) then
+      write (u, '(a)') '    <Grid Name="VelocityField" GridType="Uniform">'
+    else
+      write (u, '(a)') '    <Grid Name="ScalarField" GridType="Uniform">'
+    end if
+    write (u, '(a)') ''
+    write (u, '(a)') '      <!-- Topology: 3D grid -->'
+    write (u, '(3a)') '      <Topology TopologyType="3DCoRectMesh" Dimensions="', trim(dims), '"/>'
+    write (u, '(a)') ''
+    write (u, '(a)') '      <!-- Geometry: origin and spacing (dx, dy, dz) -->'
+    write (u, '(a)') '      <Geometry GeometryType="ORIGIN_DXDYDZ">'
+    write (u, '(a)') '        <DataItem Dimensions="3" Format="XML" Precision="4">0.0 0.0 0.0</DataItem>'
+    write (u, '(a)') '        <DataItem Dimensions="3" Format="XML" Precision="4">1.0 1.0 1.0</DataItem>'
+    write (u, '(a)') '      </Geometry>'
+    write (u, '(a)') ''
+    if (ncomp == 3) then
+      write (u, '(a)') '      <!-- Vector field associated with the grid -->'
+      write (u, '(3a)') '      <Attribute Name="', name, '" AttributeType="Vector" Center="Node">'
+      write (u, '(3a)') '        <DataItem Format="Binary" NumberType="Float" Precision="4" Dimensions="', &
+        trim(dims)//' 3', '" Endian="Little">'
+    else
+      write (u, '(a)') '      <!-- Scalar field associated with the grid -->'
+      write (u, '(3a)') '      <Attribute Name="', name, '" AttributeType="Scalar" Center="Node">'
+      write (u, '(3a)') '        <DataItem Format="Binary" NumberType="Float" Precision="4" Dimensions="', &
+        trim(dims), '" Endian="Little">'
+    end if
+    write (u, '(2a)') '          ', stem(k + 1:)//'.raw'
+    write (u, '(a)') '        </DataItem>'
+    write (u, '(a)') '      </Attribute>'
+    write (u, '(a)') ''
+    write (u, '(a)') '    </Grid>'
+    write (u, '(a)') '  </Domain>'
+    write (u, '(a)') '</Xdmf>'
+    close (u)
+  end subroutine
+
+  ! VTK XML image data with raw appended float32 data (header_vtk /
+  ! footer_vtk / print_vtk_sync, print_module.f90:22-244): extent 1..n per
+  ! axis, one point-data array of ncomp components, the appended block "_",
+  ! its byte count as int32, the values (components interleaved per node)
+  subroutine write_vti(name, var, a, d, ncomp)
+    character(len=*), intent(in) :: name, var
+    real(4), intent(in) :: a(*)
+    integer, intent(in) :: d(3), ncomp
+    character(len=1), parameter :: nl = char(10)
+    character(len=96) :: ext
+    integer :: u
+    integer(4) :: nbytes
+    write (ext, '(i0,a,i0,a,i0,a,i0,a,i0,a,i0)') 1, ' ', d(1), ' ', 1, ' ', d(2), ' ', 1, ' ', d(3)
+    nbytes = int(4, 4) * int(ncomp, 4) * int(d(1), 4) * int(d(2), 4) * int(d(3), 4)
+    open (newunit=u, file=name, status='replace', action='write', access='stream', form='unformatted')
+    write (u) '<?xml version="1.0"?>'//nl
+    write (u) '<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian">'//nl
+    write (u) '  <ImageData WholeExtent="'//trim(ext)//'">'//nl
+    write (u) '    <Piece Extent="'//trim(ext)//'">'//nl
+    write (u) '      <PointData>'//nl
+    write (u) '        <DataArray type="Float32" Name="'//var//'" NumberOfComponents="'// &
+      achar(48 + ncomp)//'" format="appended" offset="0"/>'//nl
+    write (u) '      </PointData>'//nl
+    write (u) '    </Piece>'//nl
+    write (u) '  </ImageData>'//nl
+    write (u) '  <AppendedData encoding="raw">'//nl
+    write (u) '_'
+    write (u) nbytes
+    write (u) a(1:ncomp * d(1) * d(2) * d(3))
+    write (u) nl//'  </AppendedData>'//nl//'</VTKFile>'//nl
+    close (u)
   end subroutine
 
   ! print_raw_isfluid (print_module.f90:1907-1990): the classified isfluid as
@@ -554,6 +789,214 @@ contains
     write (345) o
     close (345)
   end subroutine
+
+  subroutine write_raw4v(name, o)
+    character(len=*), intent(in) :: name
+    real(4), intent(in) :: o(:, :, :, :)
+    open (unit=345, file=name, status='replace', action='write', access='stream', form='unformatted')
+    write (345) o
+    close (345)
+  end subroutine
+
+#if defined(LAMBTEST) && defined(TWOCOMPONENT)
+  ! open_lamb / print_lamb (statistics.f90:277-471): interface positions of
+  ! the print observable r (phi with PRINTPHI, rho_r phi + (1 - phi) rho_b
+  ! otherwise) as r (1 - r)-weighted mean coordinates of each half of the line
+  ! along z through (lx/2, ly/2) and along x through (ly/2, lz/2); one line of
+  ! lamb.dat per call: step, (Lz - Lx)/(Lz + Lx), Lx, Lz, the two centres,
+  ! and the z half-axis z_int - z_cm
+  subroutine lamb_line(st)
+    integer, intent(in) :: st
+    real(db) :: zl, zln, zr, zrn, xl, xln, xr, xrn, w, r, lamb_x, lamb_z, cmx, cmz
+    integer :: gi, gj, gk
+    call download(LBF_FIELD_PHI, buf)
+#if defined(DENSRATIO) && !defined(PRINTPHI)
+    buf = rho_r * buf + (1.0_db - buf) * rho_b
+#endif
+    gi = max(1, lx / 2); gj = max(1, ly / 2)
+    zl = 0; zln = 0; zr = 0; zrn = 0
+    do gk = 1, lz
+      r = buf(gi, gj, gk); w = r * (1.0_db - r)
+      if (gk <= lz / 2) then
+        zl = zl + real(gk, db) * w; zln = zln + w
+      else
+        zr = zr + real(gk, db) * w; zrn = zrn + w
+      end if
+    end do
+    gj = max(1, ly / 2); gk = max(1, lz / 2)
+    xl = 0; xln = 0; xr = 0; xrn = 0
+    do gi = 1, lx
+      r = buf(gi, gj, gk); w = r * (1.0_db - r)
+      if (gi <= lx / 2) then
+        xl = xl + real(gi, db) * w; xln = xln + w
+      else
+        xr = xr + real(gi, db) * w; xrn = xrn + w
+      end if
+    end do
+    zl = zl / zln; zr = zr / zrn; xl = xl / xln; xr = xr / xrn
+    lamb_z = zr - zl; cmz = 0.5_db * (zr + zl)
+    lamb_x = xr - xl; cmx = 0.5_db * (xr + xl)
+    write (142, '(i8,7g16.8)') st, (lamb_z - lamb_x) / (lamb_z + lamb_x), lamb_x, lamb_z, cmx, cmz, zr - cmz
+    flush (142)
+  end subroutine
+
+  ! compare_benchmark, Lamb (statistics.f90:959-1114): the peaks of the z
+  ! half-axis in lamb.dat (local maxima above the radius, at least 2000 steps
+  ! apart), their mean spacing T_num and omega_num = 2 pi / T_num, against
+  ! Lamb's l = 2 frequency sqrt(24 sigma / (R^3 (2 rho_b + 3 rho_r))) with the
+  ! viscous correction of the two fluids; lamb_theory.dat holds R + a0 cos
+  ! of the theory at the recorded steps
+  subroutine lamb_period()
+    integer, parameter :: min_peak_dist = 2000
+    integer, allocatable :: mystep(:), pstep(:)
+    real(db), allocatable :: dosc(:), pval(:)
+    real(db) :: r_eq, mu1, mu2, om, chi, om_c, period_th, tmean, om_num, amp0, t0, a, b, c, d, e
+    integer :: maxn, myn, ios, npeaks, k, u
+    r_eq = radius
+    mu1 = rho_r * visc1
+    mu2 = rho_b * visc2
+    om = sqrt(24.0_db * sigma / (r_eq**3 * (2.0_db * rho_b + 3.0_db * rho_r)))
+    chi = ((2.0_db * 2.0_db + 1.0_db)**2 * sqrt(mu1 * mu2 * rho_r * rho_b)) / &
+          (sqrt(2.0_db) * r_eq * (2.0_db * rho_b + 3.0_db * rho_r) * (sqrt(mu1 * rho_r) + sqrt(mu2 * rho_b)))
+    om_c = om - 0.5_db * chi * sqrt(om) + 0.25_db * chi**2
+    period_th = 2.0_db * pi_greek / om_c
+    maxn = ntot / max(1, stamp_term) + 1
+    allocate (mystep(maxn), dosc(maxn), pstep(maxn), pval(maxn))
+    open (newunit=u, file='lamb.dat', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write (6, '(a)') 'Error in opening lamb.dat'
+      return
+    end if
+    myn = 0
+    do
+      if (myn >= maxn) exit
+      read (u, '(i8,7g16.8)', iostat=ios) k, a, b, c, d, e, amp0
+      if (ios /= 0) exit
+      myn = myn + 1
+      mystep(myn) = k
+      dosc(myn) = amp0
+    end do
+    close (u)
+    if (myn < 3) then
+      write (6, '(a)') 'Too few points in lamb.dat'
+      return
+    end if
+    amp0 = dosc(1) - r_eq
+    t0 = real(mystep(1), db)
+    npeaks = 0
+    do k = 2, myn - 1
+      if (dosc(k) > dosc(k - 1) .and. dosc(k) >= dosc(k + 1) .and. dosc(k) > r_eq) then
+        if (npeaks == 0) then
+          npeaks = 1; pstep(1) = mystep(k); pval(1) = dosc(k)
+        else if (mystep(k) - pstep(npeaks) >= min_peak_dist) then
+          npeaks = npeaks + 1; pstep(npeaks) = mystep(k); pval(npeaks) = dosc(k)
+        end if
+      end if
+    end do
+    open (newunit=u, file='lamb_theory.dat', status='replace', action='write')
+    do k = 1, myn
+      write (u, '(i8,g16.8)') mystep(k), r_eq + amp0 * cos(om_c * (real(mystep(k), db) - t0))
+    end do
+    close (u)
+    if (npeaks < 2) then
+      write (6, '(a)') 'Not enough peaks found to estimate omega.'
+      return
+    end if
+    tmean = real(pstep(npeaks) - pstep(1), db) / real(npeaks - 1, db)
+    om_num = 2.0_db * pi_greek / tmean
+    write (6, '(a,i10)') 'Expected number of points       = ', maxn
+    write (6, '(a,i10)') 'Number of points read           = ', myn
+    write (6, '(a,i10)') 'Number of peaks found           = ', npeaks
+    write (6, '(a,f20.10)') 'Equivalent radius               = ', r_eq
+    write (6, '(a,f20.10)') 'Theoretical period              = ', period_th
+    write (6, '(a,f20.10)') 'Theoretical omega               = ', om_c
+    write (6, '(a,f20.10)') 'Mean numerical period T_num     = ', tmean
+    write (6, '(a,f20.10)') 'Angular frequency omega_num     = ', om_num
+    write (6, '(a,f20.10)') 'zint0-zcm                       = ', dosc(1)
+    write (6, '(a,f20.10)') 'amp0                            = ', amp0
+    do k = 1, npeaks
+      write (6, '(a,i6,a,i10,a,g16.8)') 'peak ', k, ' step=', pstep(k), ' zint-zcm=', pval(k)
+    end do
+    write (6, '(a,2es20.12)') 'LAMB omega_num omega_th ', om_num, om_c
+  end subroutine
+#endif
+
+#if defined(POISEUILLE) || (defined(TWOPOISEUILLE) && defined(TWOCOMPONENT))
+  ! compare_benchmark, Poiseuille (statistics.f90:688-916): w averaged over
+  ! each x column (zero on solid nodes) against the analytic channel profile
+  ! -- the parabola fz/(2 nu) (H^2 - d^2), or the two-fluid profile of the
+  ! layers split by fcut_tanh at the centre -- with H = (lx-2)/2 under
+  ! BOUNCE_BACK, (lx-1)/2 otherwise; written to plot_poiseuille.dat or
+  ! plot_twopoiseuille.dat (x, numerical, analytic), plus a summary line
+  subroutine poiseuille_profile()
+    real(db), allocatable :: w_num(:), w_pois(:), i_num(:)
+    real(db) :: hp, xc, dist, cb, cl, cr, wl, wr, ph, en, ed
+    integer :: gi, gj, gk, u
+    character(len=32) :: fname
+    allocate (w_num(lx), w_pois(lx), i_num(lx))
+#ifdef BOUNCE_BACK
+    hp = 0.5_db * real(lx - 2, db)
+#else
+    hp = 0.5_db * real(lx - 1, db)
+#endif
+    xc = 0.5_db * real(lx + 1, db)
+#if defined(TWOPOISEUILLE)
+#ifdef DENSRATIO
+    cb = fz * hp * hp * (rho_r + rho_b) / (2.0_db * (rho_r * visc1 + rho_b * visc2))
+    cl = fz * hp * rho_b * (visc1 - visc2) / (2.0_db * visc1 * (rho_r * visc1 + rho_b * visc2))
+    cr = fz * hp * rho_r * (visc1 - visc2) / (2.0_db * visc2 * (rho_r * visc1 + rho_b * visc2))
+#else
+    cb = fz * hp * hp / (visc1 + visc2)
+    cl = fz * hp * (visc1 - visc2) / (2.0_db * visc1 * (visc1 + visc2))
+    cr = fz * hp * (visc1 - visc2) / (2.0_db * visc2 * (visc1 + visc2))
+#endif
+    fname = 'plot_twopoiseuille.dat'
+#else
+    fname = 'plot_poiseuille.dat'
+#endif
+    do gi = 1, lx
+      dist = real(gi, db) - xc
+      w_pois(gi) = 0
+      if (abs(dist) <= hp) then
+#if defined(TWOPOISEUILLE)
+        ph = fcut_tanh(dist, 0.0_db, width)
+        wl = -(fz / (2.0_db * visc1)) * dist * dist + cl * dist + cb
+        wr = -(fz / (2.0_db * visc2)) * dist * dist + cr * dist + cb
+        w_pois(gi) = wl * ph + (1.0_db - ph) * wr
+#else
+        w_pois(gi) = fz / (2.0_db * visc1) * (hp * hp - dist * dist)
+#endif
+      end if
+    end do
+    call download(LBF_FIELD_W, buf)
+    w_num = 0; i_num = 0
+    do gk = 1, lz
+      do gj = 1, ly
+        do gi = 1, lx
+          if (isf(gi, gj, gk) /= 0) w_num(gi) = w_num(gi) + buf(gi, gj, gk)
+          i_num(gi) = i_num(gi) + 1.0_db
+        end do
+      end do
+    end do
+    w_num = w_num / i_num
+    open (newunit=u, file=trim(fname), status='replace', action='write')
+#ifdef BOUNCE_BACK
+    write (u, '(3g16.8)') 1.5_db, w_num(1), w_pois(1)
+    do gi = 2, lx - 1
+      write (u, '(3g16.8)') real(gi, db), w_num(gi), w_pois(gi)
+    end do
+    write (u, '(3g16.8)') real(lx, db) - 0.5_db, w_num(lx), w_pois(lx)
+#else
+    do gi = 1, lx
+      write (u, '(3g16.8)') real(gi, db), w_num(gi), w_pois(gi)
+    end do
+#endif
+    close (u)
+    en = sqrt(sum((w_num - w_pois)**2)); ed = sqrt(sum(w_pois**2))
+    write (6, '(3a,es20.12,a,es20.12,a,es20.12)') 'POISEUILLE ', trim(fname), ' rel_l2', en / max(ed, tiny(1.0_db)), &
+      ' max_num', maxval(w_num), ' max_analytic', maxval(w_pois)
+  end subroutine
+#endif
 
   ! output/restart.raw (print_module.f90:1474-1893): the ten moments in the
   ! order 1,2,3,4,5,8,9,6,10,7 (P, u, v, w, Pxx, Pxy, Pxz, Pyy, Pyz, Pzz),

@@ -62,6 +62,19 @@ def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
         raw = setups.random_spheres(n, fill=0.15, rmin=2.0, rmax=4.0, seed=seed, margin=2)
         f = setups.two_layer_channel(n, raw, frac=0.4, width=4.0, rho_r=2.0, rho_b=1.0, axis=1)
         return p, f, raw
+    if name == "twolayer27h":  # walls in x, force in z, the two-layer start of TWOPOISEUILLE, no spheres
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, internal_obstacles=True,
+                        pbc=(0, 1, 1), visc1=0.05, visc2=0.4, rho_r=0.5, rho_b=1.5, tau_diff=0.1, sigma=0.0,
+                        width=4.0, fz=1e-6, precision=precision)
+        f = setups.two_layer_channel(n, None, frac=0.5, width=4.0, rho_r=0.5, rho_b=1.5, axis=0)
+        # velocity noise off the wall columns, so that every moment carries
+        # signal rather than rounding residue
+        rng = np.random.default_rng(seed)
+        uvw = rng.uniform(-1e-4, 1e-4, (3, n[2], n[1], n[0]))
+        uvw[:, :, :, 0] = 0.0
+        uvw[:, :, :, -1] = 0.0
+        f.update(setups._moments(f["P"], *uvw))
+        return p, f, np.ones((n[2], n[1], n[0]), dtype=np.int8)
     if name == "pois27h_1c":  # one component with walls and obstacles
         p = make_params(n, lattice=27, highorder=True, internal_obstacles=True, pbc=(1, 0, 1), visc1=0.1,
                         fx=1e-5, precision=precision)

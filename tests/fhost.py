@@ -57,6 +57,8 @@ def header(stdout: str) -> dict:
     for k, v in zip(("densratio", "incomp", "obstacles", "buoyancy", "bounce_back", "smagorinsky", "repulsive",
                      "pbc"), vals):
         out[k] = v
+    m = re.search(r"lbfast_host: bcphiflux\s+(\d+)", stdout)
+    out["bcphiflux"] = int(m.group(1)) if m else 0
     m = re.search(r"tau1 omega tau2 beta kapp\s+(.*)$", stdout, re.M)
     out["tau1"], out["omega"], out["tau2"], out["beta"], out["kapp"] = (float(x) for x in m.group(1).split())
     return out

@@ -9,6 +9,16 @@
   the radius (statistics.f90:184-275), the bounce-back
   channel keeps the Poiseuille parabola (init_conditions_module.f90:240-252).
 * A restart (output/restart.raw) continues a one-component FP32 run exactly.
+* The end-of-run benchmark comparisons of statistics.f90:688-1114: the one-
+  and two-fluid Poiseuille channels approach the analytic profiles the host
+  writes beside the numerical ones (plot_poiseuille.dat,
+  plot_twopoiseuille.dat), and the Lamb oscillation period the host derives
+  from lamb.dat equals the same peak rules applied independently to that file.
+* Output formats: XDMF sidecars describe the RAW frames, VTK image data holds
+  the same values as the RAW frame, 2-D plane frames are slices of the 3-D
+  frame of the same step (print_module.f90:2007-2660).
+* BCPHIFLUX: the host's switch reaches the library, and the phase mass of a
+  two-layer channel stays nearer its start than without it.
 """
 import os
 
@@ -23,6 +33,8 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not os.path.exists(fhost.exe("tg27h_fp32")), reason="Fortran host not built")]
 
 LAT = {"tg27h_fp32": (27, True, PREC_FP32), "tg27h_fp64": (27, True, PREC_FP64), "tg19_fp32": (19, False, PREC_FP32),
+       "pois19_fp64": (19, False, PREC_FP64), "twopois27h_fp64": (27, True, PREC_FP64),
+       "lamb27h_fp32": (27, True, PREC_FP32),
        "laplace27h_fp32": (27, True, PREC_FP32), "laplace27h_fp64": (27, True, PREC_FP64),
        "laplace27h_fp16": (27, True, PREC_FP16S), "pois19_bb_fp64": (19, False, PREC_FP64)}
 
@@ -237,3 +249,142 @@ def test_poiseuille_bounce_back_channel(tmp_path):
     got = np.stack([_f32(d.download(i)) for i in (2, 3, 4)], axis=-1)
     d.close()
     assert np.array_equal(fhost.read_frame(tmp_path, "vel", 1, n, 3), got)
+
+
+def _poiseuille_line(out):
+    line = [l for l in out.splitlines() if l.startswith("POISEUILLE ")][-1]
+    tok = line.split()
+    return {tok[i]: float(tok[i + 1]) for i in range(2, len(tok) - 1, 2)}
+
+
+def test_poiseuille_profile_and_output_formats(tmp_path):
+    """One-fluid channel with the near-wall closure (walls in x, force in z):
+    the x profile written to plot_poiseuille.dat approaches the parabola with
+    H = (lx-1)/2; the frames of the run come as RAW with XDMF sidecars, as
+    VTK image data and as 2-D planes."""
+    variant = "pois19_fp64"
+    n, nu, fz, steps = (34, 4, 8), 1.0 / 6.0, 5e-6, 6000
+    sim = _sim(n, steps, pbc_x=0, stamp=steps, stamp2D=steps, lvtk=True, nplanes=2)
+    out = _run(variant, tmp_path, sim, {"visc1": nu, "fz": fz, "ndir": [1, 3], "npoint": [9, 5]}, )
+    res = _poiseuille_line(out)
+    print(f"[fortran host]   poiseuille {res}", flush=True)
+    prof = np.loadtxt(tmp_path / "plot_poiseuille.dat")
+    assert prof.shape == (n[0], 3)
+    assert res["rel_l2"] < 0.05
+    # the profile the host wrote is the column mean of its own last frame,
+    # with the solid wall columns counted as zero
+    vel = fhost.read_frame(tmp_path, "vel", 1, n, 3).astype(np.float64)
+    assert np.allclose(prof[1:-1, 1], vel[..., 2].mean(axis=(0, 1))[1:-1], rtol=1e-6, atol=1e-12)
+    assert prof[0, 1] == 0 and prof[-1, 1] == 0
+    # XDMF sidecars name their RAW files and the grid, slowest axis first
+    x = (tmp_path / "output" / "out_vel_000001.xdmf").read_text()
+    assert "out_vel_000001.raw" in x and f'Dimensions="{n[2]} {n[1]} {n[0]} 3"' in x
+    assert 'AttributeType="Vector"' in x and 'Endian="Little"' in x
+    # VTK image data: the appended block holds the RAW frame's bytes
+    raw = (tmp_path / "output" / "out_vel_000001.raw").read_bytes()
+    vti = (tmp_path / "output" / "out_vel_000001.vti").read_bytes()
+    head, rest = vti.split(b'<AppendedData encoding="raw">\n', 1)
+    assert b'WholeExtent="1 34 1 4 1 8"' in head and b'NumberOfComponents="3"' in head
+    assert rest[:1] == b"_"
+    nbytes = int(np.frombuffer(rest[1:5], dtype=np.int32)[0])
+    assert nbytes == len(raw) and rest[5:5 + nbytes] == raw
+    assert rest[5 + nbytes:].startswith(b"\n  </AppendedData>")
+    # 2-D planes: x = 9 and z = 5 slices of the 3-D frame of the same step
+    rho = fhost.read_frame(tmp_path, "rho", 1, n)
+    px = np.fromfile(tmp_path / "output" / "out_1_9_rho_000001.raw", dtype=np.float32).reshape(n[2], n[1])
+    pz = np.fromfile(tmp_path / "output" / "out_3_5_vel_000001.raw", dtype=np.float32).reshape(n[1], n[0], 3)
+    assert np.array_equal(px, rho[:, :, 8])
+    assert np.array_equal(pz, fhost.read_frame(tmp_path, "vel", 1, n, 3)[4])
+    assert (tmp_path / "output" / "out_1_9_rho_000001.xdmf").exists()
+
+
+def test_two_fluid_poiseuille_profile(tmp_path):
+    """twopoiseuille.inp's physics at 34 x 4 x 8: two layers split at the
+    channel centre (nu 0.05 / 0.4, rho 0.5 / 1.5, contact angle 90), walls in
+    x, force in z.  The host's profile equals the FP64 oracle's run of the
+    same start; both sit at rel-L2 0.16 from the reference's sharp-interface
+    analytic profile, the diffuse interface (w = 4) spanning a quarter of
+    the channel."""
+    import oracle
+    variant = "twopois27h_fp64"
+    n, steps = (34, 4, 8), 8000
+    fl = {"fz": 1e-6, "tau_diff": 0.1, "width": 4.0, "sigma": 0.0, "visc1": 0.05, "visc2": 0.4,
+          "rho_r": 0.5, "rho_b": 1.5, "wettab_r": 90.0}
+    out = _run(variant, tmp_path, _sim(n, steps, pbc_x=0, stamp=steps, stamp_term=steps), fl, timeout=300)
+    res = _poiseuille_line(out)
+    prof = np.loadtxt(tmp_path / "plot_twopoiseuille.dat")
+    assert prof.shape == (n[0], 3)
+    # the same start in the oracle: walls by classification, the two-layer
+    # phi with the solid-node averages of init_conditions_module.f90:468-542,
+    # and the analytic w the host starts from
+    p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, internal_obstacles=True,
+                    pbc=(0, 1, 1), visc1=0.05, visc2=0.4, rho_r=0.5, rho_b=1.5, tau_diff=0.1, sigma=0.0, width=4.0,
+                    fz=1e-6, precision=PREC_FP64)
+    isf = oracle.classify_isfluid(p, None)
+    shape = (n[2], n[1], n[0])
+    phi = np.broadcast_to(setups.fcut_tanh(np.arange(1, n[0] + 1) - 0.5 * (n[0] + 1), 0.0, 4.0), shape).copy()
+    phi = setups.solid_phi_average(phi, isf, 27)
+    w0 = np.where(np.abs(isf) == 1, np.broadcast_to(prof[:, 2], shape), 0.0)
+    f = {k: np.zeros(shape) for k in ("P", "u", "v", "pxx", "pyy", "pxy", "pxz", "pyz")}
+    f.update({"w": w0, "pzz": w0 * w0, "phi": phi})
+    o = oracle.Oracle(p)
+    o.load(f, isf)
+    o.step(steps)
+    w_orc = np.where(isf != 0, o.get_field(4), 0.0).mean(axis=(0, 1))
+    e_orc = float(np.abs(prof[:, 1] - w_orc).max() / np.abs(w_orc).max())
+    print(f"[fortran host]   two-fluid poiseuille {res}; host vs oracle profile {e_orc:.2e}", flush=True)
+    assert e_orc < 5e-8
+    assert res["rel_l2"] < 0.2
+    assert (tmp_path / "output" / "out_press_000001.xdmf").exists()
+
+
+def test_lamb_period_analysis(tmp_path):
+    """The Lamb test with lamb.inp's droplet (an ellipsoid with semi-axes 2R,
+    R, R; R = 20, width 5, sigma = 0.03) in half of lamb.inp's box, 128^3,
+    with nu = 0.01: lamb.dat every stamp_term steps, and at the end the peaks
+    of the z half-axis at least 2000 steps apart, their mean spacing and
+    omega_num, which the same rules applied here to lamb.dat reproduce.  The
+    oscillation of this droplet is slower than the 2000-step peak spacing,
+    so every peak is found."""
+    variant = "lamb27h_fp32"
+    n, steps, every = (128, 128, 128), 12000, 20
+    fl = {"tau_diff": 0.1, "radius": 20.0, "width": 5.0, "center": [64.0, 64.0, 64.0], "sigma": 0.03,
+          "visc1": 0.01, "visc2": 0.01, "rho_r": 1.0, "rho_b": 1.0, "wettab_r": 90.0}
+    out = _run(variant, tmp_path, _sim(n, steps, stamp=steps, stamp_term=every, lraw=False), fl, timeout=300)
+    rows = np.loadtxt(tmp_path / "lamb.dat")
+    assert rows.shape[0] == steps // every + 1 and np.all(np.isfinite(rows))
+    line = [l for l in out.splitlines() if l.startswith("LAMB omega_num omega_th")]
+    assert line, out[-2000:]
+    om_num, om_th = (float(x) for x in line[0].split()[-2:])
+    step, dosc = rows[:, 0].astype(int), rows[:, 6]
+    peaks = []
+    for i in range(1, len(dosc) - 1):
+        if dosc[i] > dosc[i - 1] and dosc[i] >= dosc[i + 1] and dosc[i] > 20.0:
+            if not peaks or step[i] - peaks[-1] >= 2000:
+                peaks.append(step[i])
+    assert len(peaks) >= 3
+    # every local maximum of the record is a peak: none falls inside the
+    # 2000-step exclusion of the one before it
+    maxima = [int(step[i]) for i in range(1, len(dosc) - 1)
+              if dosc[i] > dosc[i - 1] and dosc[i] >= dosc[i + 1] and dosc[i] > 20.0 and step[i] < 9500]
+    assert maxima == [q for q in peaks if q < 9500]
+    tmean = (peaks[-1] - peaks[0]) / (len(peaks) - 1)
+    print(f"[fortran host]   lamb peaks {peaks} omega_num {om_num:.6f} theory {om_th:.6f}", flush=True)
+    assert abs(om_num - 2 * np.pi / tmean) < 1e-6 * om_num
+    assert (tmp_path / "lamb_theory.dat").exists()
+
+
+def test_bcphiflux_switch_reaches_the_library(tmp_path):
+    drift = {}
+    for variant in ("twopois27h_fp64", "twopois27h_bcphi_fp64"):
+        wd = tmp_path / variant
+        wd.mkdir()
+        n, steps = (34, 4, 8), 400
+        fl = {"fz": 1e-6, "tau_diff": 0.1, "width": 4.0, "sigma": 0.01, "visc1": 0.05, "visc2": 0.4,
+              "rho_r": 0.5, "rho_b": 1.5, "wettab_r": 90.0}
+        out = _run(variant, wd, _sim(n, steps, pbc_x=0, stamp=steps, stamp_term=20, lraw=False), fl)
+        assert fhost.header(out)["bcphiflux"] == (1 if "bcphi" in variant else 0)
+        mass = [r["phase_mass"] for r in fhost.stats(out, "LAPLACE")]
+        drift[variant] = max(abs(m - mass[0]) for m in mass) / mass[0]
+    print(f"[fortran host]   phase-mass drift {drift}", flush=True)
+    assert drift["twopois27h_bcphi_fp64"] < drift["twopois27h_fp64"]

@@ -114,6 +114,15 @@ def test_obstacles_fast(name, prec):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("n", [(34, 4, 8), (20, 3, 5), (70, 6, 2)])
+@pytest.mark.parametrize("path", ["fast", "staged"])
+def test_obstacles_thin_domains(n, path):
+    """Two-component runs with walls on domains thinner than a row tile and a
+    z chunk: ny of 2..6, nz of 2..8 (the Fortran host's channel decks)."""
+    res, bad = run_pair("twolayer27h", PREC_FP64, n, 20, path=path)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("name", ["tg27h_force", "drop27h", "chan27h"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_MIXED])
 def test_stage_path(name, prec):
