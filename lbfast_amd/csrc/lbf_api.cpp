@@ -203,12 +203,9 @@ static void fill_devparams(lbf_ctx* c) {
   f.tau_diff = (float)q.tau_diff; f.sharp_c = (float)q.sharp_c; f.beta = (float)q.beta; f.kapp = (float)q.kapp;
   f.omega = (float)q.omega; f.wettab = (float)q.wettab;
   f.width = (float)q.width; f.a_rep = (float)q.a_rep; f.q_th = (float)q.q_th; f.cos_opp = (float)q.cos_opp;
-  // FP16S/BF16S: power-of-two scales that put typical deviations (P ~ 1e-4..1e-2,
-  // u ~ 1e-3..1e-1, Pi - u u ~ 1e-7..1e-3) in the normal range of binary16
-  const double dflt[10] = {16384.0, 1024.0, 1024.0, 1024.0, 1048576.0, 1048576.0, 1048576.0,
-                           1048576.0, 1048576.0, 1048576.0};
+  // FP16S/BF16S: the given scales, or the power-of-two defaults (def_scale)
   for (int i = 0; i < 10; ++i) {
-    const double s = p.store_scale[i] > 0 ? p.store_scale[i] : dflt[i];
+    const double s = p.store_scale[i] > 0 ? p.store_scale[i] : (double)def_scale(i);
     const bool devc = c->prec == LBF_PREC_FP16S || c->prec == LBF_PREC_BF16S;
     d.scale[i] = devc ? (float)s : 1.0f;
     d.invscale[i] = devc ? (float)(1.0 / s) : 1.0f;
@@ -928,7 +925,13 @@ static int batch_refresh(lbf_batch* b) {
 static int batch_enqueue(lbf_batch* b, int par) {
   const int nd = (int)b->ctx.size();
   if (b->two) BHIPCHK(b, launch_phase_prep_batch(b->ptab[par], nd, b->ptotal[par], b->prec, b->stream));
-  BHIPCHK(b, launch_fused_batch(b->ftab[par], nd, b->ftotal[par], b->prec, b->lattice, b->ho, b->two, b->ty, b->stream));
+  int kprec = b->prec;
+  if (kprec == LBF_PREC_FP16S || kprec == LBF_PREC_BF16S) {
+    bool ds = true;
+    for (lbf_ctx* c : b->ctx) ds = ds && default_scales(c->dp);
+    if (ds) kprec |= PREC_DEFSCALE;
+  }
+  BHIPCHK(b, launch_fused_batch(b->ftab[par], nd, b->ftotal[par], kprec, b->lattice, b->ho, b->two, b->ty, b->stream));
   if (b->prec == LBF_PREC_FP16S || b->prec == LBF_PREC_BF16S) BHIPCHK(b, launch_tick(b->ctr, nd, b->stream));
   return LBF_OK;
 }

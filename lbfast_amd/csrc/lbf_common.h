@@ -291,6 +291,31 @@ template <> struct Aux<PREC_FP16S> { typedef float T; };
 template <> struct Aux<PREC_BF16S> { typedef float T; };
 template <> struct Aux<PREC_MIXED> { typedef float T; };
 
+// The 16-bit codecs' default scales: powers of two that put typical
+// deviations (P ~ 1e-4..1e-2, u ~ 1e-3..1e-1, Pi - u u ~ 1e-7..1e-3) in the
+// normal range of binary16.  A kernel instantiated for PREC | PREC_DEFSCALE
+// has them compiled in as instruction literals; with the scales as kernel
+// arguments the twenty values occupy scalar registers, which the fused
+// kernel then spills to vector-register lanes and reloads every plane.
+constexpr int PREC_DEFSCALE = 8;
+__host__ __device__ constexpr int prec_base(int P) { return P & 7; }
+__host__ __device__ constexpr float def_scale(int i) { return i == 0 ? 16384.0f : (i < 4 ? 1024.0f : 1048576.0f); }
+template <> struct Codec<PREC_FP16S | PREC_DEFSCALE> : Codec<PREC_FP16S> {};
+template <> struct Codec<PREC_BF16S | PREC_DEFSCALE> : Codec<PREC_BF16S> {};
+template <> struct Aux<PREC_FP16S | PREC_DEFSCALE> { typedef float T; };
+template <> struct Aux<PREC_BF16S | PREC_DEFSCALE> { typedef float T; };
+// scale and inverse scale of field i (i a constant after unrolling)
+template <int P>
+__device__ __forceinline__ float scl(const DevParams& dp, int i) {
+  if constexpr ((P & PREC_DEFSCALE) != 0) return def_scale(i);
+  else return dp.scale[i];
+}
+template <int P>
+__device__ __forceinline__ float iscl(const DevParams& dp, int i) {
+  if constexpr ((P & PREC_DEFSCALE) != 0) return 1.0f / def_scale(i);
+  else return dp.invscale[i];
+}
+
 // The ten moments of one node in compute precision.
 template <class R> struct Mom {
   R P, u, v, w, xx, yy, zz, xy, xz, yz;
@@ -380,14 +405,14 @@ __device__ __forceinline__ void store_mom_b(typename Codec<P>::T* plane0, size_t
     // each field's random bits: the FP32 bits of the next field of the same
     // group (cyclic), XOR the seed
     auto sr = [&](float x, float xn) { return C::srx(x, __float_as_uint(xn) ^ rnd); };
-    const float x0 = o.P * dp.scale[0], x1 = o.u * dp.scale[1], x2 = o.v * dp.scale[2], x3 = o.w * dp.scale[3];
+    const float x0 = o.P * scl<P>(dp, 0), x1 = o.u * scl<P>(dp, 1), x2 = o.v * scl<P>(dp, 2), x3 = o.w * scl<P>(dp, 3);
     fst(plane0, fs, 0, pb, boff, sr(x0, x1));
     const T tu = sr(x1, x2), tv = sr(x2, x3), tw = sr(x3, x0);
-    const R ur = C::dec(tu, dp.invscale[1]), vr = C::dec(tv, dp.invscale[2]), wr = C::dec(tw, dp.invscale[3]);
+    const R ur = C::dec(tu, iscl<P>(dp, 1)), vr = C::dec(tv, iscl<P>(dp, 2)), wr = C::dec(tw, iscl<P>(dp, 3));
     fst(plane0, fs, 1, pb, boff, tu); fst(plane0, fs, 2, pb, boff, tv); fst(plane0, fs, 3, pb, boff, tw);
-    const float x4 = (o.xx - ur * ur) * dp.scale[4], x5 = (o.yy - vr * vr) * dp.scale[5];
-    const float x6 = (o.zz - wr * wr) * dp.scale[6], x7 = (o.xy - ur * vr) * dp.scale[7];
-    const float x8 = (o.xz - ur * wr) * dp.scale[8], x9 = (o.yz - vr * wr) * dp.scale[9];
+    const float x4 = (o.xx - ur * ur) * scl<P>(dp, 4), x5 = (o.yy - vr * vr) * scl<P>(dp, 5);
+    const float x6 = (o.zz - wr * wr) * scl<P>(dp, 6), x7 = (o.xy - ur * vr) * scl<P>(dp, 7);
+    const float x8 = (o.xz - ur * wr) * scl<P>(dp, 8), x9 = (o.yz - vr * wr) * scl<P>(dp, 9);
     fst(plane0, fs, 4, pb, boff, sr(x4, x5));
     fst(plane0, fs, 5, pb, boff, sr(x5, x6));
     fst(plane0, fs, 6, pb, boff, sr(x6, x7));

@@ -374,7 +374,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef typename C::T T;
   typedef typename Aux<PREC>::T TA;
   constexpr int NW = TY + 2;  // row waves (TY output rows + 2 halo rows)
-  constexpr bool PF = PREC == PREC_FP32 || PREC == PREC_FP16S || PREC == PREC_BF16S;
+  constexpr int PB = prec_base(PREC);
+  constexpr bool PF = PB == PREC_FP32 || PB == PREC_FP16S || PB == PREC_BF16S;
   // GI: grad phi and lap phi formed here from a phi tile (fused_grad_inline)
   constexpr bool GI = PF && TWO && INLINE && !OB;
   typedef Slots<TWO, INLINE, GI> SL;
@@ -592,11 +593,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
     {
       NodeIn<R> in;
       Mom<R> m;
-      m.P = C::dec(cur.m[0], dp.invscale[0]); m.u = C::dec(cur.m[1], dp.invscale[1]);
-      m.v = C::dec(cur.m[2], dp.invscale[2]); m.w = C::dec(cur.m[3], dp.invscale[3]);
-      m.xx = C::dec(cur.m[4], dp.invscale[4]); m.yy = C::dec(cur.m[5], dp.invscale[5]);
-      m.zz = C::dec(cur.m[6], dp.invscale[6]); m.xy = C::dec(cur.m[7], dp.invscale[7]);
-      m.xz = C::dec(cur.m[8], dp.invscale[8]); m.yz = C::dec(cur.m[9], dp.invscale[9]);
+      m.P = C::dec(cur.m[0], iscl<PREC>(dp, 0)); m.u = C::dec(cur.m[1], iscl<PREC>(dp, 1));
+      m.v = C::dec(cur.m[2], iscl<PREC>(dp, 2)); m.w = C::dec(cur.m[3], iscl<PREC>(dp, 3));
+      m.xx = C::dec(cur.m[4], iscl<PREC>(dp, 4)); m.yy = C::dec(cur.m[5], iscl<PREC>(dp, 5));
+      m.zz = C::dec(cur.m[6], iscl<PREC>(dp, 6)); m.xy = C::dec(cur.m[7], iscl<PREC>(dp, 7));
+      m.xz = C::dec(cur.m[8], iscl<PREC>(dp, 8)); m.yz = C::dec(cur.m[9], iscl<PREC>(dp, 9));
       if (C::dev) {
         m.xx += m.u * m.u; m.yy += m.v * m.v; m.zz += m.w * m.w;
         m.xy += m.u * m.v; m.xz += m.u * m.w; m.yz += m.v * m.w;
@@ -619,9 +620,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
           lap = (R)cur.g[3];
           dvt = (R)cur.g[4];
           if (dp.dens) {
-            uox = C::dec(cur.uo[0], dp.invscale[1]);
-            uoy = C::dec(cur.uo[1], dp.invscale[2]);
-            uoz = C::dec(cur.uo[2], dp.invscale[3]);
+            uox = C::dec(cur.uo[0], iscl<PREC>(dp, 1));
+            uoy = C::dec(cur.uo[1], iscl<PREC>(dp, 2));
+            uoz = C::dec(cur.uo[2], iscl<PREC>(dp, 3));
           }
         }
         R tFx, tFy, tFz;
@@ -637,10 +638,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
           if constexpr (TWO) {
             if (dp.dens) {
               T* uo = (T*)a.uout + (size_t)zw * ny * 3 * fs;
-              T eu = C::enc(m.u, dp.scale[1]), ev = C::enc(m.v, dp.scale[2]), ew = C::enc(m.w, dp.scale[3]);
+              T eu = C::enc(m.u, scl<PREC>(dp, 1)), ev = C::enc(m.v, scl<PREC>(dp, 2)), ew = C::enc(m.w, scl<PREC>(dp, 3));
               if constexpr (C::dev) {
                 const uint32_t st = nseed ^ (uint32_t)zw * 0x27D4EB2Fu;
-                const float xu = m.u * dp.scale[1], xv = m.v * dp.scale[2], xw_ = m.w * dp.scale[3];
+                const float xu = m.u * scl<PREC>(dp, 1), xv = m.v * scl<PREC>(dp, 2), xw_ = m.w * scl<PREC>(dp, 3);
                 eu = C::srx(xu, __float_as_uint(xv) ^ st);
                 ev = C::srx(xv, __float_as_uint(xw_) ^ st);
                 ew = C::srx(xw_, __float_as_uint(xu) ^ st);
@@ -919,6 +920,10 @@ static hipError_t batch_prec(const FusedBatchEntry* tab, int ndom, int total, in
 hipError_t launch_fused_batch(const FusedBatchEntry* dev_tab, int ndom, int total_tiles, int prec, int lattice,
                               bool highorder, bool two, int ty, hipStream_t s) {
   switch (prec) {
+    case PREC_FP16S | PREC_DEFSCALE:
+      return batch_prec<PREC_FP16S | PREC_DEFSCALE>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
+    case PREC_BF16S | PREC_DEFSCALE:
+      return batch_prec<PREC_BF16S | PREC_DEFSCALE>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
     case PREC_FP64: return batch_prec<PREC_FP64>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
     case PREC_FP32: return batch_prec<PREC_FP32>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
     case PREC_MIXED: return batch_prec<PREC_MIXED>(dev_tab, ndom, total_tiles, lattice, highorder, two, ty, s);
@@ -928,14 +933,23 @@ hipError_t launch_fused_batch(const FusedBatchEntry* dev_tab, int ndom, int tota
   return hipErrorInvalidValue;
 }
 
+bool default_scales(const DevParams& dp) {
+  for (int i = 0; i < 10; ++i)
+    if (dp.scale[i] != def_scale(i)) return false;
+  return true;
+}
+
 hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s) {
   if (inline_moments) {
+    const bool ds = default_scales(a.dp);
     switch (prec) {
       case PREC_FP64: return launch_ty<PREC_FP64, true>(a, s);
       case PREC_FP32: return launch_ty<PREC_FP32, true>(a, s);
       case PREC_MIXED: return launch_ty<PREC_MIXED, true>(a, s);
-      case PREC_FP16S: return launch_ty<PREC_FP16S, true>(a, s);
-      case PREC_BF16S: return launch_ty<PREC_BF16S, true>(a, s);
+      case PREC_FP16S:
+        return ds ? launch_ty<PREC_FP16S | PREC_DEFSCALE, true>(a, s) : launch_ty<PREC_FP16S, true>(a, s);
+      case PREC_BF16S:
+        return ds ? launch_ty<PREC_BF16S | PREC_DEFSCALE, true>(a, s) : launch_ty<PREC_BF16S, true>(a, s);
     }
   } else {
     switch (prec) {

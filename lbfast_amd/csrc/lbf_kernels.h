@@ -67,7 +67,10 @@ struct PrepBatchEntry {
 };
 
 hipError_t launch_fused(const FusedArgs& a, int prec, bool inline_moments, hipStream_t s);
+// every 16-bit scale equals def_scale(i): the kernels with compiled-in scales apply
+bool default_scales(const DevParams& dp);
 int fused_tile_count(const FusedArgs& a);
+// prec may carry PREC_DEFSCALE (every domain with the default 16-bit scales)
 hipError_t launch_fused_batch(const FusedBatchEntry* dev_tab, int ndom, int total_tiles, int prec, int lattice,
                               bool highorder, bool two, int ty, hipStream_t s);
 int prep_tile_count(const StageArgs& a, int prec, int tz);

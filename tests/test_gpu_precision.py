@@ -107,6 +107,36 @@ def test_sixteen_bit_steps_against_fp64(n, prec, tol_u):
     assert ephi < 1e-4
 
 
+@pytest.mark.parametrize("prec,tol_u", [(PREC_FP16S, 3e-3), (PREC_BF16S, 3e-2)])
+def test_given_scales_against_default_scales(prec, tol_u):
+    """The default scales are compiled into their own kernels; any other
+    store_scale takes the kernels that read the scales as arguments.  With
+    every scale doubled the run stays as close to FP64 as the default one
+    (the stochastic rounding draws differ, so not bit for bit), and a
+    download decodes with the given scales."""
+    n, steps = (40, 24, 20), 50
+    print(f"[precision] given scales prec={prec} n={n} steps={steps}", flush=True)
+    res = {}
+    for tag, q in (("fp64", PREC_FP64), ("default", prec), ("doubled", prec)):
+        p, f = _droplet(n, q)
+        if tag == "doubled":
+            for i in range(10):
+                p.store_scale[i] = 2.0 * SCALES[i]
+        d = hip.Domain(p)
+        d.load(f)
+        if tag == "doubled":
+            u0 = d.download(2)
+            half_ulp = 2.0 ** -11 if prec == PREC_FP16S else 2.0 ** -8  # of the significand
+            assert np.abs(u0 - f["u"]).max() <= half_ulp * np.abs(f["u"]).max() + 1e-12
+        d.step(steps)
+        res[tag] = np.stack([d.download(i) for i in (2, 3, 4)])
+        d.close()
+    e = {k: np.linalg.norm(res[k] - res["fp64"]) / np.linalg.norm(res["fp64"]) for k in ("default", "doubled")}
+    print(f"[precision]   u error default {e['default']:.2e} doubled {e['doubled']:.2e}", flush=True)
+    assert e["default"] < tol_u and e["doubled"] < tol_u
+    assert e["doubled"] < 2.0 * e["default"] + 1e-5
+
+
 def test_porous_channel_against_fp64():
     n, steps = (64, 48, 32), 2000
     raw = setups.random_spheres(n, fill=0.25, rmin=3.0, rmax=6.0, seed=11, margin=2)
