@@ -15,7 +15,14 @@ direction) fails it:
   * the phase update leaves a constant phi at rest unchanged;
   * isfluid classification matches a brute-force restatement;
   * the wetting ghost update reduces to phi_ghost = phi_donor at 90 degrees
-    and to phi_donor - |grad_t| cot(theta) 4 phi(1-phi) otherwise, on a flat wall.
+    and to phi_donor - |grad_t| cot(theta) 4 phi(1-phi) otherwise, on a flat wall,
+    and to phi_donor - sqrt 2 |t| cot(theta) 4 phi(1-phi) on a 45-degree
+    staircase wall (normal, donor and tangential part derived by hand);
+  * the surface-tension force of the moments stage, (4 beta phi (phi-1)
+    (phi-1/2) - kappa lap phi) grad phi with beta = 12 sigma/w and
+    kappa = 1.5 sigma w, vanishes on the equilibrium tanh profile of width w
+    (the profile the chemical potential's zero fixes: xi^2 = 2 kappa/beta =
+    (w/2)^2), with a residual of second order in 1/w.
 """
 import numpy as np
 import pytest
@@ -316,3 +323,86 @@ def test_wetting_ghost_flat_wall(angle):
     assert np.abs(got - expect).max() < 1e-13
     # deeper solid nodes have no fluid neighbour along the normal: unchanged
     assert np.abs(o.get_field(FIELD_PHI)[:, :, 0] - ghost0).max() == 0.0
+
+
+@pytest.mark.parametrize("angle", [90.0, 60.0, 135.0])
+def test_wetting_ghost_45_degree_wall(angle):
+    """A 45-degree staircase wall, solid where i + j <= 14 (0-based, z
+    periodic).  Hand derivation for a staircase node (i + j = 14) and the
+    layer below it (i + j = 13): the weighted sum over the non-solid lattice
+    neighbours points along -(1, 1, 0) (19/6 and 7/6 times it), so the unit
+    wall normal is wn = -(1, 1, 0)/sqrt 2; of the fluid neighbours the
+    diagonal (1, 1, 0) scores highest ((e . -wn) |e|^-1 = 1 against 0.816 for
+    (1, 1, +-1) and 0.707 for the axes), so the donor sits sqrt 2 away along
+    the normal with back-direction bs = (-1, -1, 0).  With donor gradient g,
+    tangential part t = g - (g . wn) wn and b = t - |t| cot(theta') wn,
+    b . bs = -sqrt 2 |t| cot(theta'), so the ghost value is
+    phi_donor - sqrt 2 |t| cot(theta') 4 phi_g (1 - phi_g) -- the flat-wall
+    rule with the donor distance sqrt 2.  Nodes with i + j <= 12 have no
+    non-solid neighbour and keep their value."""
+    shape = (16, 16, 4)
+    p = _two_params(shape, internal_obstacles=True, pbc=(0, 0, 1), wettab_r=angle)
+    k, j, i = np.meshgrid(*(np.arange(n) for n in (4, 16, 16)), indexing="ij")
+    raw = np.where(i + j <= 14, 0, 1).astype(np.int8)
+    isf = oracle.classify_isfluid(p, raw)
+    f = _zeros(shape)
+    f["phi"] = np.full(i.shape, 0.4)
+    o = oracle.Oracle(p)
+    o.load(f, isf)
+    g = np.array([0.003, 0.001, -0.002])  # normal, in-plane tangential and z parts
+    mg = np.linalg.norm(g)
+    for c in range(3):
+        o.set_field(FIELD_AUX[c], np.full(i.shape, g[c] / mg))
+    o.set_field(FIELD_AUX[3], np.full(i.shape, mg))
+    ghost0 = 0.25
+    phis = np.where(isf == 0, ghost0, 0.4)
+    o.set_field(FIELD_PHI, phis)
+    o.begin_step()
+    o.stage(STAGE_PHI_INT_BOUNDARY)
+    got = o.get_field(FIELD_PHI)
+    wn = -np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    t = g - g.dot(wn) * wn
+    th = (180.0 - angle) * np.pi / 180.0
+    cot = np.cos(th) / np.sin(th)
+    expect = np.clip(0.4 - np.sqrt(2.0) * np.linalg.norm(t) * cot * 4 * ghost0 * (1 - ghost0), 0, 1)
+    inside = (i >= 3) & (i <= 12) & (j >= 3) & (j <= 12)
+    for layer in (14, 13):
+        sel = inside & (i + j == layer)
+        assert sel.sum() > 0 and (isf[sel] == 0).all()
+        assert np.abs(got[sel] - expect).max() < 1e-14, (layer, got[sel][:3], expect)
+    sel = inside & (i + j <= 12)
+    assert np.abs(got[sel] - ghost0).max() == 0.0
+
+
+def test_surface_tension_vanishes_on_the_equilibrium_profile():
+    """A planar slab phi = fcut_tanh(|x - xc|, 96, w) at rest with equal
+    densities: the force the moments stage stores is the surface-tension term
+    alone.  Relative to the size of its bulk part, 4 beta phi(phi-1)(phi-1/2)
+    grad phi, it falls as w^-2 (the stencils' truncation error): below 1 % at
+    w = 16, by a factor 4 +- 1 per doubling of w.  A wrong beta or kappa
+    coefficient, a sign slip between the two terms or a wrong Laplacian leaves
+    an O(1) residual that does not fall with w."""
+    ratio = {}
+    for w in (8.0, 16.0, 32.0):
+        n, sigma = (384, 3, 3), 0.01
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True, visc1=0.1, visc2=0.1,
+                        rho_r=1.0, rho_b=1.0, tau_diff=0.1, sigma=sigma, width=w, precision=PREC_FP64)
+        shape = (n[2], n[1], n[0])
+        x = np.arange(1, n[0] + 1, dtype=float)
+        phi1 = setups.fcut_tanh(np.abs(x - 192.5), 96.0, w)
+        f = _zeros(n)
+        f["phi"] = np.broadcast_to(phi1, shape).copy()
+        o = oracle.Oracle(p)
+        o.load(f)
+        o.begin_step()
+        for st in (STAGE_NORM, STAGE_DIV_THETA_N, STAGE_MOMENTS):
+            o.stage(st)
+        fx = o.get_field(FIELD_FORCE[0])[0, 0]
+        assert np.abs(o.get_field(FIELD_FORCE[1])).max() == 0.0
+        beta = 12.0 * sigma / w
+        gx = 0.5 * (np.roll(phi1, -1) - np.roll(phi1, 1))
+        bulk = 4.0 * beta * phi1 * (phi1 - 1.0) * (phi1 - 0.5) * gx
+        ratio[w] = np.abs(fx).max() / np.abs(bulk).max()
+    assert ratio[16.0] < 0.01, ratio
+    assert 3.0 < ratio[8.0] / ratio[16.0] < 5.0, ratio
+    assert 3.0 < ratio[16.0] / ratio[32.0] < 5.0, ratio

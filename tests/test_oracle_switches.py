@@ -1,6 +1,9 @@
 """Oracle pins for the optional switches: BOUNCE_BACK, SMAGORINSKI and
 REPULSIVE_FLUX.
 
+* The default near-wall closure (LB_int_boundary): a body-force channel
+  relaxes to an exact parabola with the bulk curvature -F/nu and its zeros
+  within 0.05 of the solid layers' centres.
 * BOUNCE_BACK (lb_cuda_fused_module.f90:20-424): halfway bounce-back conserves
   the fluid mass exactly, and a body-force driven channel relaxes to the
   Poiseuille parabola with the walls half a node inside the solid layer
@@ -85,6 +88,36 @@ def test_bounce_back_poiseuille():
     # the walls sit half a node inside the solid layer: the nodes next to the
     # walls carry the parabola's value there, not zero
     assert u[1] > 0.5 * want[1]
+
+
+def test_near_wall_closure_channel_is_parabolic():
+    """The default near-wall closure (LB_int_boundary, lb_cuda_boundary_module.
+    f90:117-212) in a body-force channel, walls in y, run to steady state: the
+    fluid profile, the closure's nodes next to the walls included, is a
+    parabola to round-off with the bulk curvature -F/nu, and its zeros (the
+    effective walls) lie within 0.05 of the solid layers' centres.  A closure
+    that mis-sets the wall nodes' populations bends the profile near the
+    walls or moves its zeros by O(1)."""
+    for ly, nu in ((18, 1.0 / 6.0), (18, 0.05)):
+        n, fx = (3, ly, 3), 1e-6
+        p = make_params(n, lattice=19, internal_obstacles=True, pbc=(1, 0, 1), visc1=nu, fx=fx,
+                        precision=PREC_FP64)
+        isf = oracle.classify_isfluid(p, None)
+        shape = (n[2], n[1], n[0])
+        f = {k: np.zeros(shape) for k in ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")}
+        o = oracle.Oracle(p)
+        o.load(f, isf)
+        o.step(int(3 * ly * ly / nu))
+        u = o.get_field(2)[0, :, 0]
+        o.step(50)
+        assert np.abs(o.get_field(2)[0, :, 0] - u).max() < 1e-12 * u.max()  # steady
+        gj = np.arange(1, ly + 1, dtype=float)
+        inner = slice(1, ly - 1)
+        c = np.polyfit(gj[inner], u[inner], 2)
+        assert np.abs(np.polyval(c, gj[inner]) - u[inner]).max() < 1e-9 * u.max()
+        assert abs(2.0 * c[0] / (-fx / nu) - 1.0) < 1e-9
+        lo, hi = sorted(np.roots(c).real)
+        assert abs(lo - 1.0) < 0.05 and abs(hi - ly) < 0.05, (lo, hi)
 
 
 def _smag_omega(pneq_scale, nu=0.02):
