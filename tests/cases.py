@@ -46,6 +46,22 @@ def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
         f = setups.droplet(n, R, 3.0, [n[0] / 2 + 0.5, n[1] / 2 + 0.5, n[2] / 2 + 0.5], 0.01, 1.0, 0.1,
                            noise=1e-4, seed=seed)
         return p, f, None
+    if name in ("drop27h_bulk", "bubble27h_bulk"):
+        # a droplet (or, inverted, a bubble) off centre whose phi is exactly 0
+        # or 1 away from the interface
+        R = 8.0
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True,
+                        visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=0.1, tau_diff=0.1, sigma=0.01, width=3.0,
+                        precision=precision)
+        f = setups.droplet(n, R, 3.0, [28.5, n[1] / 2 + 0.5, n[2] / 2 + 0.5], 0.01, 1.0, 0.1,
+                           noise=1e-4, seed=seed)
+        phi = np.where(f["phi"] < 1e-4, 0.0, np.where(f["phi"] > 1.0 - 1e-4, 1.0, f["phi"]))
+        if name == "bubble27h_bulk":
+            phi = 1.0 - phi
+        rhophi = 1.0 * phi + (1.0 - phi) * 0.1
+        f["phi"] = phi
+        f["P"] = phi * 0.02 / R / (rhophi / 3.0)
+        return p, f, None
     if name == "drop27h_nodens":
         R = n[0] / 4.0
         p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=False,

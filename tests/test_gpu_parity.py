@@ -123,6 +123,21 @@ def test_obstacles_thin_domains(n, path):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("name", ["drop27h_bulk", "bubble27h_bulk"])
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP64])
+def test_bulk_planes(name, prec):
+    """A droplet (phi = 1 inside) or a bubble (phi = 0 inside) whose phi is
+    exactly 0 or 1 away from the interface (theta = phi (1 - phi) n exactly
+    zero there, grad phi exactly zero inside whole tiles), in a box with two
+    x tiles, two y tiles and 40 planes, FP32 and FP64 against the oracle."""
+    n = (96, 40, 40)
+    p, f, raw = case(name, n, prec)
+    bulk = 0.0 if name == "drop27h_bulk" else 1.0  # the value of the far tile
+    assert (f["phi"][:, :, 60:] == bulk).all() and (f["phi"] == bulk).mean() > 0.7
+    res, bad = run_pair(name, prec, n, 20)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("name", ["tg27h_force", "drop27h", "chan27h"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_MIXED])
 def test_stage_path(name, prec):

@@ -137,6 +137,28 @@ def test_given_scales_against_default_scales(prec, tol_u):
     assert e["doubled"] < 2.0 * e["default"] + 1e-5
 
 
+@pytest.mark.parametrize("name", ["drop27h_bulk", "bubble27h_bulk"])
+@pytest.mark.parametrize("prec,tol_u", [(PREC_FP16S, 3e-3), (PREC_BF16S, 3e-2)])
+def test_bulk_planes_against_fp64(name, prec, tol_u):
+    """The droplet (or bubble) with exact bulk values away from the
+    interface: the 16-bit run stays within the droplet bounds of the FP64 run."""
+    n, steps = (96, 40, 40), 50
+    print(f"[precision] bulk planes {name} prec={prec} n={n} steps={steps}", flush=True)
+    res = {}
+    for q in (PREC_FP64, prec):
+        p, f, _ = cases.case(name, n, q)
+        d = hip.Domain(p)
+        d.load(f)
+        d.step(steps)
+        res[q] = (np.stack([d.download(i) for i in (2, 3, 4)]), d.download(FIELD_PHI))
+        d.close()
+    (u64, phi64), (u16, phi16) = res[PREC_FP64], res[prec]
+    eu = np.linalg.norm(u16 - u64) / np.linalg.norm(u64)
+    ephi = np.linalg.norm(phi16 - phi64) / np.linalg.norm(phi64)
+    print(f"[precision]   u {eu:.2e} phi {ephi:.2e}", flush=True)
+    assert eu < tol_u and ephi < 1e-4
+
+
 def test_porous_channel_against_fp64():
     n, steps = (64, 48, 32), 2000
     raw = setups.random_spheres(n, fill=0.25, rmin=3.0, rmax=6.0, seed=11, margin=2)
