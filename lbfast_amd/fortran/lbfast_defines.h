@@ -8,7 +8,8 @@
  * PRC (8|4), MIXEDPRC (PRC 8 arithmetic, 4-byte storage), STORE_FP16 /
  * STORE_BF16 (16-bit scaled-deviation storage, FP32 arithmetic); test case:
  * TAYLORGREEN, LAPLACE, LAMBTEST, POISEUILLE, TWOPOISEUILLE (default: the
- * droplet of the generic initial condition); output: WRITEPRESS, PRINTPHI.
+ * droplet of the generic initial condition); output: WRITEPRESS, PRINTPHI,
+ * DOXDMF (an XDMF description beside every RAW frame).
  * DOBENCHMARK (on, as in the reference; -DNO_DOBENCHMARK turns it off) sets
  * the benchmark defaults that an input deck then overrides: a MYSIDE^3 box
  * (MYSIDE 512), 100 steps, RAW output every 100 steps. */
