@@ -28,6 +28,8 @@
  *   repulsive_flux_normal_cuda      (driver:882-910)   -> lbf_stage(LBF_STAGE_REPULSIVE_FLUX)
  *   phi_sum_count_cuda + apply_lagrangian_phi_cuda (driver:2011-2077, BCPHIFLUX)
  *                                                      -> lbf_stage(LBF_STAGE_PHI_LAGRANGE)
+ *   open-boundary loops of bcs_mesoscopic_hfields / _phifields
+ *     (boundary_cds_module.f90:57-1311, 1353-1664)     -> lbf_stage(LBF_STAGE_OPEN_BC_H / _PHI)
  *   fused_LB_cuda under BOUNCE_BACK (driver:1464-1465, fused_LB_kernel_bb)
  *                                                      -> lbf_stage(LBF_STAGE_FUSED), bounce_back = 1
  *   exchange_*_intpbc (mpi_module.f90:2203-3524)       -> in-kernel periodic wrap (no call)
@@ -46,7 +48,7 @@
 extern "C" {
 #endif
 
-#define LBF_ABI_VERSION 1
+#define LBF_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define LBF_OK          0
@@ -112,7 +114,17 @@ extern "C" {
 #define LBF_STAGE_THINFILM_SCAN     8  /* thinfilm_scan_mark_kernel (REPULSIVE_FLUX) */
 #define LBF_STAGE_REPULSIVE_FLUX    9  /* repulsive_flux_normal_kernel     */
 #define LBF_STAGE_PHI_LAGRANGE     10  /* phi_sum_count_kernel + apply_lagrangian_phi_kernel
-                                          (BCPHIFLUX), after PHI_INT_BOUNDARY */
+                                          (BCPHIFLUX), after PHI_INT_BOUNDARY (and OPEN_BC_PHI);
+                                          with openbc the target adds the phase flux
+                                          through the open faces' first fluid layers */
+#define LBF_STAGE_OPEN_BC_PHI      11  /* open faces, phase field: the wall layer takes
+                                          phi of the node inside it
+                                          (boundary_cds_module.f90:1353-1664) */
+#define LBF_STAGE_OPEN_BC_H        12  /* open faces, moments: after LB_INT_BOUNDARY, the
+                                          populations a first-layer node receives from
+                                          solid nodes are replaced by the regularised
+                                          populations of the face's pressure / velocity
+                                          (boundary_cds_module.f90:57-1311) */
 
 /* Parameters: the &simulation / &fluid namelists (LBFAST.f90:32-60) that
  * the time step reads, the defines.h switches, and the derived quantities
@@ -145,6 +157,16 @@ typedef struct lbf_params {
   /* REPULSIVE_FLUX namelist values (LBFAST.f90:57); the scan window is 3 (LBFAST.f90:221) */
   double a_rep, q_th, cos_opp_t;
   double reserved_d;
+  /* open boundaries (&simulation, LBFAST.f90:33-37; internal obstacles only):
+   * openbc = 1 applies them on every non-periodic face.  Faces in the order
+   * x(1), x(2), y(1), y(2), z(1), z(2) (low, high).  openbc_type: 2 the
+   * node's own pressure and velocity, 1 the face's openbc_press with the
+   * node's velocity, 0 the node's pressure with the face's velocity
+   * (openbc_u, openbc_v, openbc_w). */
+  int32_t openbc;
+  int32_t openbc_type[6];
+  int32_t reserved_i2;
+  double openbc_press[6], openbc_u[6], openbc_v[6], openbc_w[6];
 } lbf_params;
 
 typedef struct lbf_ctx lbf_ctx;

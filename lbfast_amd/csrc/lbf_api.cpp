@@ -7,6 +7,7 @@
 // isfluid with a 1-node halo.  All launches go to one HIP stream.
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -87,6 +88,11 @@ static void* dalloc(lbf_ctx* c, size_t bytes, hipError_t* e) {
 }
 
 extern "C" {
+
+// the layout the Python and Fortran mirrors of lbf_params assume
+static_assert(sizeof(lbf_params) == 544, "lbf_params: 18 + 8 int32 and 33 + 24 double");
+static_assert(offsetof(lbf_params, openbc) == 320 && offsetof(lbf_params, openbc_press) == 352,
+              "lbf_params: open-boundary block");
 
 int32_t lbf_abi_version(void) { return LBF_ABI_VERSION; }
 
@@ -272,6 +278,17 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     return fail(nullptr, LBF_EINVAL, "BCPHIFLUX requires TWOCOMPONENT and INTERNAL_OBSTACLES");
   if (p.repulsive_flux && (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S))
     return fail(nullptr, LBF_EINVAL, "REPULSIVE_FLUX runs the stage path: FP64, FP32 or MIXED storage");
+  if (p.openbc) {
+    // the open-boundary loops sit in bcs_mesoscopic_hfields / _phifields,
+    // compiled only with INTERNAL_OBSTACLES (boundary_cds_module.f90:49-1311)
+    if (!p.internal_obstacles) return fail(nullptr, LBF_EINVAL, "open boundaries require INTERNAL_OBSTACLES");
+    if (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S)
+      return fail(nullptr, LBF_EINVAL, "open boundaries run the stage path: FP64, FP32 or MIXED storage");
+    for (int f = 0; f < 6; ++f)
+      if (p.openbc_type[f] < 0 || p.openbc_type[f] > 2)
+        return fail(nullptr, LBF_EINVAL, "openbc_type %d of face %d: 0 (velocity), 1 (pressure) or 2 (local)",
+                    p.openbc_type[f], f);
+  }
   if (p.visc1 <= 0 || (p.twocomponent && p.visc2 <= 0)) return fail(nullptr, LBF_EINVAL, "viscosity must be > 0");
   if (p.twocomponent && p.densratio && (p.rho_r <= 0 || p.rho_b <= 0)) return fail(nullptr, LBF_EINVAL, "rho_r, rho_b must be > 0");
   if (lbf_params_derive(&p) != LBF_OK) return fail(nullptr, LBF_EINVAL, "width must be non-zero for TWOCOMPONENT");
@@ -549,6 +566,14 @@ static StageArgs stage_args(lbf_ctx* c) {
   a.rep_mask = c->rep_mask;
   a.isfh = c->isfh;
   a.dp = c->dp;
+  a.obc = c->p.openbc != 0 && c->p.internal_obstacles != 0;
+  for (int f = 0; f < 6; ++f) {
+    a.obc_type[f] = c->p.openbc_type[f];
+    a.obc_val[f][0] = c->p.openbc_press[f];
+    a.obc_val[f][1] = c->p.openbc_u[f];
+    a.obc_val[f][2] = c->p.openbc_v[f];
+    a.obc_val[f][3] = c->p.openbc_w[f];
+  }
   return a;
 }
 
@@ -603,6 +628,7 @@ int lbf_init_state(lbf_ctx* c) {
     StageArgs a = stage_args(c);
     if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, true, c->stream));
     HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+    if (a.obc) HIPCHK(c, launch_stage(LBF_STAGE_OPEN_BC_PHI, a, c->prec, c->stream));
     if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -639,6 +665,7 @@ static int staged_step(lbf_ctx* c) {
   if (two) {
     HIPCHK(c, launch_stage(LBF_STAGE_NORM, a, c->prec, c->stream));
     if (obst) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+    if (a.obc) HIPCHK(c, launch_stage(LBF_STAGE_OPEN_BC_PHI, a, c->prec, c->stream));
     if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
     HIPCHK(c, launch_stage(LBF_STAGE_DIV_THETA_N, a, c->prec, c->stream));
     if (c->dp.rep) {
@@ -651,6 +678,9 @@ static int staged_step(lbf_ctx* c) {
   if (rc) return rc;
   if (two) HIPCHK(c, launch_stage(LBF_STAGE_UPDATE_PHI, a, c->prec, c->stream));
   if (obst && !c->p.bounce_back) HIPCHK(c, launch_stage(LBF_STAGE_LB_INT_BOUNDARY, a, c->prec, c->stream));
+  // the open faces follow LB_int_boundary in bcs_mesoscopic_hfields, also
+  // under BOUNCE_BACK (boundary_cds_module.f90:49-57)
+  if (a.obc) HIPCHK(c, launch_stage(LBF_STAGE_OPEN_BC_H, a, c->prec, c->stream));
   c->cur ^= 1;
   c->ucur ^= 1;
   return LBF_OK;
@@ -664,9 +694,11 @@ int lbf_step(lbf_ctx* c, int32_t nsteps) {
   HIPCHK(c, hipEventRecord(c->t0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
     ++c->step;
-    if (c->dp.rep) {
+    if (c->dp.rep || (c->p.openbc && c->p.internal_obstacles)) {
       // the repulsive scan reads the normals of compute_norm's aux set, which
-      // the fused fast path does not form: run the reference's stage order
+      // the fused fast path does not form, and the open faces read the
+      // post-moments state S, which the fast path keeps in registers: run
+      // the reference's stage order
       int rc = staged_step(c);
       if (rc) return rc;
       continue;
@@ -747,6 +779,12 @@ int lbf_stage(lbf_ctx* c, int id) {
     case LBF_STAGE_THINFILM_SCAN:
     case LBF_STAGE_REPULSIVE_FLUX:
       if (!c->dp.rep) return LBF_OK;
+      break;
+    case LBF_STAGE_OPEN_BC_PHI:
+      if (!two || !obst || !c->p.openbc) return LBF_OK;
+      break;
+    case LBF_STAGE_OPEN_BC_H:
+      if (!obst || !c->p.openbc) return LBF_OK;
       break;
     case LBF_STAGE_PHI_LAGRANGE: {
       if (!c->lag) return LBF_OK;

@@ -55,99 +55,6 @@
 
 namespace lbf {
 
-// ---------------------------------------------------------------------------
-// Post-collision populations f[cidx(p,q,r)] of one node.
-template <int LAT, bool HO, class R>
-__device__ __forceinline__ void collide(const NodeIn<R>& n, R* f) {
-  const R om1 = (R)1 - n.om;
-  const R u = n.u, v = n.v, w = n.w;
-  // order <= 2 Hermite coefficients (post-collision, with half forcing)
-  const R A000 = n.P;
-  const R A100 = u + (R)0.5 * n.ax, A010 = v + (R)0.5 * n.ay, A001 = w + (R)0.5 * n.az;
-  const R A200 = u * u + om1 * n.xx + u * n.ax;
-  const R A020 = v * v + om1 * n.yy + v * n.ay;
-  const R A002 = w * w + om1 * n.zz + w * n.az;
-  const R A110 = u * v + om1 * n.xy + (R)0.5 * (u * n.ay + v * n.ax);
-  const R A101 = u * w + om1 * n.xz + (R)0.5 * (u * n.az + w * n.ax);
-  const R A011 = v * w + om1 * n.yz + (R)0.5 * (v * n.az + w * n.ay);
-  if constexpr (LAT == 27) {
-    R A[27];  // A[a + 3b + 9c]
-#define AI(a, b, c) A[(a) + 3 * (b) + 9 * (c)]
-    AI(0, 0, 0) = A000;
-    AI(1, 0, 0) = A100; AI(0, 1, 0) = A010; AI(0, 0, 1) = A001;
-    AI(2, 0, 0) = A200; AI(0, 2, 0) = A020; AI(0, 0, 2) = A002;
-    AI(1, 1, 0) = A110; AI(1, 0, 1) = A101; AI(0, 1, 1) = A011;
-    if constexpr (HO) {
-      const R pxx = n.xx, pyy = n.yy, pzz = n.zz, pxy = n.xy, pxz = n.xz, pyz = n.yz;
-      const R uu = u * u, vv = v * v, ww = w * w;
-      // E_k and N_k of fused:668-702 (k = 10..26), N_k transcribed as written
-      AI(2, 1, 0) = uu * v + om1 * ((R)2 * pxy * u + pxx * v);
-      AI(2, 0, 1) = uu * w + om1 * ((R)2 * pxz * u + pxx * w);
-      AI(1, 2, 0) = u * vv + om1 * (pyy * u + (R)2 * pxy * v);
-      AI(1, 0, 2) = u * ww + om1 * (pzz * u + (R)2 * pxz * w);
-      AI(0, 1, 2) = v * ww + om1 * (pzz * v + (R)2 * pyz * w);
-      AI(0, 2, 1) = vv * w + om1 * ((R)2 * pyz * v + pyy * w);
-      AI(1, 1, 1) = u * v * w + om1 * (pyz * u + pxz * v + pxy * w);
-      AI(2, 2, 0) = uu * vv + om1 * (pyy * uu + (R)2 * v * ((R)3 * pxy * u + pxx * v));
-      AI(2, 0, 2) = uu * ww + om1 * (pzz * uu + (R)2 * w * ((R)3 * pxz * u + pxx * w));
-      AI(0, 2, 2) = vv * ww + om1 * (pzz * vv + (R)2 * w * ((R)3 * pyz * v + pyy * w));
-      AI(1, 1, 2) = u * v * ww + om1 * (pzz * u * v + w * ((R)3 * pyz * u + (R)3 * pxz * v + (R)2 * pxy * w));
-      AI(1, 2, 1) = u * vv * w + om1 * ((R)2 * pyz * u * v + pxz * vv + (R)2 * pyy * u * w + (R)4 * pxy * v * w);
-      AI(2, 1, 1) = uu * v * w + om1 * (pyz * uu + (R)2 * pxz * u * v + (R)4 * pxy * u * w + (R)2 * pxx * v * w);
-      AI(2, 1, 2) = uu * v * ww +
-                    om1 * (pzz * uu * v + (R)3 * w * (pyz * uu + (R)2 * pxz * u * v + (R)2 * pxy * u * w + pxx * v * w));
-      AI(2, 2, 1) = uu * vv * w +
-                    om1 * ((R)2 * u * v * (pyz * u + pxz * v) + (R)2 * pyy * uu * w + v * ((R)10 * pxy * u + (R)3 * pxx * v) * w);
-      AI(1, 2, 2) = u * vv * ww +
-                    om1 * (pzz * u * vv + (R)3 * w * ((R)2 * pyz * u * v + pxz * vv + pyy * u * w + (R)2 * pxy * v * w));
-      AI(2, 2, 2) = uu * vv * ww +
-                    om1 * (pzz * uu * vv + w * ((R)6 * u * v * (pyz * u + pxz * v) + (R)3 * pyy * uu * w +
-                                                (R)2 * v * ((R)7 * pxy * u + (R)2 * pxx * v) * w));
-    } else {
-      AI(2, 1, 0) = 0; AI(2, 0, 1) = 0; AI(1, 2, 0) = 0; AI(1, 0, 2) = 0; AI(0, 1, 2) = 0; AI(0, 2, 1) = 0;
-      AI(1, 1, 1) = 0; AI(2, 2, 0) = 0; AI(2, 0, 2) = 0; AI(0, 2, 2) = 0; AI(1, 1, 2) = 0; AI(1, 2, 1) = 0;
-      AI(2, 1, 1) = 0; AI(2, 1, 2) = 0; AI(2, 2, 1) = 0; AI(1, 2, 2) = 0; AI(2, 2, 2) = 0;
-    }
-#undef AI
-    // x pass: A[a][b][c] -> B[p][b][c]
-    R B[27];
-#pragma unroll
-    for (int bc = 0; bc < 9; ++bc) recon3(A[3 * bc + 0], A[3 * bc + 1], A[3 * bc + 2], B[3 * bc + 0], B[3 * bc + 1], B[3 * bc + 2]);
-    // y pass: B[p][b][c] -> A[p][q][c]
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int pp = 0; pp < 3; ++pp)
-        recon3(B[pp + 0 + 9 * c], B[pp + 3 + 9 * c], B[pp + 6 + 9 * c], A[pp + 0 + 9 * c], A[pp + 3 + 9 * c], A[pp + 6 + 9 * c]);
-    // z pass: A[p][q][c] -> f[p][q][r]
-#pragma unroll
-    for (int pq = 0; pq < 9; ++pq) recon3(A[pq], A[pq + 9], A[pq + 18], f[pq], f[pq + 9], f[pq + 18]);
-  } else {
-    // D3Q19: f_l = w_l [A0 + 3 c.A1 + 4.5 sum_ab (c_a c_b - delta/3) A2_ab]
-#pragma unroll
-    for (int r = -1; r <= 1; ++r)
-#pragma unroll
-      for (int q = -1; q <= 1; ++q)
-#pragma unroll
-        for (int p = -1; p <= 1; ++p) {
-          const int c = cidx(p, q, r);
-          if (!present<LAT>(p, q, r)) { f[c] = 0; continue; }
-          const R wl = (R)weight<LAT>(p, q, r);
-          R s = A000;
-          if (p) s += (R)(3 * p) * A100;
-          if (q) s += (R)(3 * q) * A010;
-          if (r) s += (R)(3 * r) * A001;
-          s += (R)(4.5 * (p * p - 1.0 / 3.0)) * A200;
-          s += (R)(4.5 * (q * q - 1.0 / 3.0)) * A020;
-          s += (R)(4.5 * (r * r - 1.0 / 3.0)) * A002;
-          if (p * q) s += (R)(9 * p * q) * A110;
-          if (p * r) s += (R)(9 * p * r) * A101;
-          if (q * r) s += (R)(9 * q * r) * A011;
-          f[c] = wl * s;
-        }
-  }
-}
-
 // Whole-wave lane shifts through DPP (wave_shr:1 / wave_shl:1, a VALU
 // operand modifier, no LDS round trip).  With bound_ctrl off, the lane whose
 // source lies outside the wave (lane 0 for shr, lane 63 for shl) keeps `fill`.

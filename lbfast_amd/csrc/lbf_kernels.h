@@ -43,6 +43,12 @@ struct StageArgs {
   int8_t* rep_mask;  // REPULSIVE_FLUX: nx*ny*nz
   const int8_t* isfh;
   DevParams dp;
+  // open boundaries (openbc = 1): faces x(1) x(2) y(1) y(2) z(1) z(2), each
+  // with its type (0 velocity, 1 pressure, 2 the node's own values) and its
+  // pressure and velocity
+  int obc;
+  int obc_type[6];
+  double obc_val[6][4];
 };
 
 // The fused kernel forms grad phi and lap phi itself from phi (the prep

@@ -596,6 +596,15 @@ __global__ void __launch_bounds__(BS) k_phi_bnd(StageArgs a) {
   // the reference skips a donor with a zero (1-based) coordinate, i.e. one in
   // the lower halo (boundary:395); 0-based that is -1
   if (!have || dx == -1 || dy == -1 || dz == -1) return;
+  // a donor past an upper non-periodic face (isfluid 1 there) lies in the
+  // reference's halo, which nothing writes on such an axis: phi and its
+  // gradient read zero, so the ghost is 0 + 0 (the donor search picks such a
+  // node only where two walls meet: on a plane wall the halo's and the
+  // fluid's pulls on the wall normal cancel)
+  if ((!dp.pbc[0] && dx == nx) || (!dp.pbc[1] && dy == ny) || (!dp.pbc[2] && dz == nz)) {
+    ph[n] = (TA)0;
+    return;
+  }
   const int dxw = wrapg(dx, nx), dyw = wrapg(dy, ny), dzw = wrapg(dz, nz);
   const size_t dn = nidx(dxw, dyw, dzw, nx, ny);
   const R pf = (R)ph[dn];
@@ -911,6 +920,124 @@ __global__ void __launch_bounds__(BS) k_repulsive_flux(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Open boundaries (openbc = 1, internal obstacles; boundary_cds_module.f90).
+// Face f = 2*axis + side of every non-periodic axis.  Its wall is the solid
+// layer at coordinate 0 (side 0) or n-1 (side 1) along the axis; the
+// moments closure works on the first fluid layer, coordinate 1 or n-2.  The
+// reference's face loops skip the face's edges: the two other coordinates
+// run over 1 .. n-2 (gj > 1 .and. gj < ly, 0-based here).
+__device__ __forceinline__ bool open_face_node(const DevParams& dp, int f, int x, int y, int z, int depth) {
+  const int ax = f >> 1, side = f & 1;
+  if (dp.pbc[ax]) return false;
+  const int c[3] = {x, y, z}, n[3] = {dp.nx, dp.ny, dp.nz};
+  if (c[ax] != (side ? n[ax] - 1 - depth : depth)) return false;
+  for (int b = 0; b < 3; ++b)
+    if (b != ax && (c[b] < 1 || c[b] > n[b] - 2)) return false;
+  return true;
+}
+
+// Phase field (boundary_cds_module.f90:1353-1664): a wall node of an open
+// face takes phi of the node next to it inside the domain.  The faces do not
+// overlap (edges are skipped) and no node read is written.
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_open_phi(StageArgs a) {
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  int x, y, z;
+  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  for (int f = 0; f < 6; ++f) {
+    if (!open_face_node(dp, f, x, y, z, 0)) continue;
+    const int ax = f >> 1, d = (f & 1) ? -1 : 1;
+    TA* phi = (TA*)a.phiS;
+    phi[nidx(x, y, z, dp.nx, dp.ny)] =
+        phi[nidx(x + (ax == 0) * d, y + (ax == 1) * d, z + (ax == 2) * d, dp.nx, dp.ny)];
+  }
+}
+
+// Moments (boundary_cds_module.f90:57-1311), after LB_int_boundary, on the
+// first fluid layer (isfluid -1) of each open face: for every population that
+// a node receives from a solid node, take out the near-wall closure's
+// population, w_l P + (1 - omega) fneq_l with the non-equilibrium stress
+// Pi - cs^2 P - u u of the node, and put in the regularised population of the
+// face state -- P and u of the node, or the face's pressure (type 1) or
+// velocity (type 0) in their place -- with the same stress and, under
+// HIGHORDER, the E_k / N_k terms.  The outgoing second moments of the removed
+// population are taken without the cs^2 shift (fpost c_a c_b), those of the
+// added one with it, as the reference writes them.  S is the post-moments
+// state, O the output of the fused kernel and LB_int_boundary; a node on two
+// faces (next to an edge of the layer) is corrected for both, in face order.
+template <int PREC>
+__global__ void __launch_bounds__(BS) k_open_h(StageArgs a) {
+  typedef Codec<PREC> C;
+  typedef typename C::R R;
+  typedef typename C::T T;
+  typedef typename Aux<PREC>::T TA;
+  const DevParams& dp = a.dp;
+  int x, y, z;
+  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
+  bool on = false;
+  for (int f = 0; f < 6; ++f) on = on || open_face_node(dp, f, x, y, z, 1);
+  if (!on || isf_at(a.isfh, x, y, z, nx, ny, nz) != -1) return;
+  const size_t n = nidx(x, y, z, nx, ny);
+  const int sx = dp.sx;
+  const size_t n10 = sbase(10, x, y, z, sx, ny);
+  const Phys<R>& ph = dp.ph<R>();
+  const R cssq = (R)(1.0 / 3.0), ONE = 1;
+  const Mom<R> s = load_mom<PREC>((const T*)a.S + n10, sx, 0u, dp);
+  Mom<R> o = load_mom<PREC>((const T*)a.O + n10, sx, 0u, dp);
+  R om;
+  if (dp.twocomp) {
+    const R phl = (R)((const TA*)a.phiS)[n];
+    const R rhophi = dp.dens ? ph.rho_r * phl + (ONE - phl) * ph.rho_b : ONE;
+    const R visc = (ph.rho_r * ph.visc1 * phl + (ONE - phl) * ph.visc2 * ph.rho_b) / rhophi;
+    om = ONE / (visc * (R)3 + (R)0.5);
+  } else {
+    om = ph.omega;
+  }
+  NodeIn<R> sub{}, add{};
+  sub.P = s.P;
+  sub.xx = s.xx - cssq * s.P - s.u * s.u; sub.yy = s.yy - cssq * s.P - s.v * s.v;
+  sub.zz = s.zz - cssq * s.P - s.w * s.w;
+  sub.xy = s.xy - s.u * s.v; sub.xz = s.xz - s.u * s.w; sub.yz = s.yz - s.v * s.w;
+  sub.om = om;
+  R fs[27], fa[27];
+  if (dp.lattice == 27) collide<27, false, R>(sub, fs);
+  else collide<19, false, R>(sub, fs);
+  for (int f = 0; f < 6; ++f) {
+    if (!open_face_node(dp, f, x, y, z, 1)) continue;
+    add = sub;
+    add.u = s.u; add.v = s.v; add.w = s.w;
+    const int type = a.obc_type[f];
+    if (type == 1) {
+      add.P = (R)a.obc_val[f][0];
+    } else if (type != 2) {
+      add.u = (R)a.obc_val[f][1]; add.v = (R)a.obc_val[f][2]; add.w = (R)a.obc_val[f][3];
+    }
+    if (dp.lattice == 19) collide<19, false, R>(add, fa);
+    else if (dp.highorder) collide<27, true, R>(add, fa);
+    else collide<27, false, R>(add, fa);
+    const int nl = dp.lattice == 27 ? 27 : 19;
+    for (int l = 1; l < nl; ++l) {
+      if (isf_at(a.isfh, x - cEX[l], y - cEY[l], z - cEZ[l], nx, ny, nz) != 0) continue;
+      const R ex = (R)cEX[l], ey = (R)cEY[l], ez = (R)cEZ[l];
+      const int ci = cidx(cEX[l], cEY[l], cEZ[l]);
+      const R fr = fs[ci];
+      o.P = o.P - fr;
+      o.u = o.u - fr * ex; o.v = o.v - fr * ey; o.w = o.w - fr * ez;
+      o.xx = o.xx - fr * ex * ex; o.yy = o.yy - fr * ey * ey; o.zz = o.zz - fr * ez * ez;
+      o.xy = o.xy - fr * ex * ey; o.xz = o.xz - fr * ex * ez; o.yz = o.yz - fr * ey * ez;
+      const R fp = fa[ci];
+      o.P = o.P + fp;
+      o.u = o.u + fp * ex; o.v = o.v + fp * ey; o.w = o.w + fp * ez;
+      o.xx = o.xx + fp * (ex * ex - cssq); o.yy = o.yy + fp * (ey * ey - cssq); o.zz = o.zz + fp * (ez * ez - cssq);
+      o.xy = o.xy + fp * ex * ey; o.xz = o.xz + fp * ex * ez; o.yz = o.yz + fp * ey * ez;
+    }
+  }
+  store_mom<PREC>((T*)a.O + n10, sx, 0u, o, dp);
+}
+
+// ---------------------------------------------------------------------------
 template <int PREC>
 static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
   const size_t n = a.dp.nn;
@@ -924,6 +1051,8 @@ static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
     case 7: hipLaunchKernelGGL(k_lb_int_bnd<PREC>, grid, block, 0, s, a); break;
     case 8: hipLaunchKernelGGL(k_thinfilm_scan<PREC>, grid, block, 0, s, a); break;
     case 9: hipLaunchKernelGGL(k_repulsive_flux<PREC>, grid, block, 0, s, a); break;
+    case 11: hipLaunchKernelGGL(k_open_phi<PREC>, grid, block, 0, s, a); break;
+    case 12: hipLaunchKernelGGL(k_open_h<PREC>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1159,9 +1288,10 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 template <int PREC>
 __global__ void __launch_bounds__(BS) k_phi_lag_sum(StageArgs a, double* partial) {
   typedef typename Aux<PREC>::T TA;
-  __shared__ double sh[2][BS];
+  typedef typename Codec<PREC>::T T;
+  __shared__ double sh[3][BS];
   const DevParams& dp = a.dp;
-  double sum = 0, cnt = 0;
+  double sum = 0, cnt = 0, flux = 0;
   for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < dp.nn; t += (size_t)gridDim.x * BS) {
     int x, y, z;
     node_of(t, dp, x, y, z);
@@ -1170,26 +1300,45 @@ __global__ void __launch_bounds__(BS) k_phi_lag_sum(StageArgs a, double* partial
     const double ph = (double)((const TA*)a.phiS)[t];
     sum += ph;
     if (ph > 0.5 && ph < 0.9) cnt += 1.0;
+    if (a.obc && isf == -1) {
+      // with open boundaries, the phase carried through each open face's
+      // first fluid layer (boundary_cds_module.f90:1667-1862): + u phi on
+      // the low face, - u phi on the high face, u the velocity along the
+      // face's axis in S; no edge is skipped here
+      const int c[3] = {x, y, z}, nn[3] = {dp.nx, dp.ny, dp.nz};
+      for (int ax = 0; ax < 3; ++ax) {
+        if (dp.pbc[ax]) continue;
+        const bool lo = c[ax] == 1, hi = c[ax] == nn[ax] - 2;
+        if (!lo && !hi) continue;
+        const T* S = (const T*)a.S + sbase(10, x, y, z, dp.sx, dp.ny);
+        const double ua = (double)Codec<PREC>::dec(S[(size_t)(1 + ax) * dp.sx], dp.invscale[1 + ax]);
+        if (lo) flux += ua * ph;
+        if (hi) flux -= ua * ph;
+      }
+    }
   }
   sh[0][threadIdx.x] = sum;
   sh[1][threadIdx.x] = cnt;
+  sh[2][threadIdx.x] = flux;
   __syncthreads();
   for (int st = BS / 2; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st) {
       sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
       sh[1][threadIdx.x] += sh[1][threadIdx.x + st];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + st];
     }
     __syncthreads();
   }
-  if (threadIdx.x < 2) partial[(size_t)blockIdx.x * 2 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 3) partial[(size_t)blockIdx.x * 3 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void k_phi_lag_final(const double* partial, int nb, double* lag, int init) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double sum = 0, cnt = 0;
+  double sum = 0, cnt = 0, change = 0;
   for (int b = 0; b < nb; ++b) {
-    sum += partial[2 * b];
-    cnt += partial[2 * b + 1];
+    sum += partial[3 * b];
+    cnt += partial[3 * b + 1];
+    change += partial[3 * b + 2];
   }
   if (init) {
     lag[0] = sum;  // global_phi_sum_ini (init_conditions_module.f90:469-490)
@@ -1203,7 +1352,10 @@ __global__ void k_phi_lag_final(const double* partial, int nb, double* lag, int 
   // corr = dphi / real(global_count): the count is converted to a default
   // (single precision) real, as the reference's real() without a kind does.
   // No counted node (an empty interface band) gives no correction.
-  lag[3] = cnt > 0 ? (lag[0] - sum) / (double)(float)cnt : 0.0;
+  // With open boundaries the target adds this step's flux through the open
+  // faces (dphi = (global_phi_sum_ini + global_phi_change) - global_phi_sum,
+  // boundary_cds_module.f90:1864-1870; the flux is reset every step).
+  lag[3] = cnt > 0 ? ((lag[0] + change) - sum) / (double)(float)cnt : 0.0;
 }
 
 template <int PREC>

@@ -3,7 +3,8 @@
 ! The C-ABI replaces the reference's CUDA driver module
 ! (lb_cuda_driver_module.f90); the mapping of its subroutines to these
 ! functions is listed in include/lbfast_hip.h.  lbf_params mirrors the C
-! struct member by member (18 int32, then 33 real(8); 320 bytes).
+! struct member by member (18 int32, 33 real(8), then the open-boundary
+! block: 8 int32 and 24 real(8); 544 bytes).
 module lbfast_hip
   use iso_c_binding
   implicit none
@@ -14,7 +15,8 @@ module lbfast_hip
   integer(c_int), parameter :: LBF_STAGE_NORM = 1, LBF_STAGE_PHI_INT_BOUNDARY = 2, &
                                LBF_STAGE_DIV_THETA_N = 3, LBF_STAGE_MOMENTS = 4, LBF_STAGE_FUSED = 5, &
                                LBF_STAGE_UPDATE_PHI = 6, LBF_STAGE_LB_INT_BOUNDARY = 7, &
-                               LBF_STAGE_THINFILM_SCAN = 8, LBF_STAGE_REPULSIVE_FLUX = 9
+                               LBF_STAGE_THINFILM_SCAN = 8, LBF_STAGE_REPULSIVE_FLUX = 9, &
+                               LBF_STAGE_PHI_LAGRANGE = 10, LBF_STAGE_OPEN_BC_PHI = 11, LBF_STAGE_OPEN_BC_H = 12
   integer(c_int), parameter :: LBF_FIELD_P = 1, LBF_FIELD_U = 2, LBF_FIELD_V = 3, LBF_FIELD_W = 4, &
                                LBF_FIELD_PXX = 5, LBF_FIELD_PHI = 11, LBF_FIELD_RHOPHI = 40
   integer(c_int), parameter :: LBF_DTYPE_F64 = 0, LBF_DTYPE_F32 = 1
@@ -26,6 +28,9 @@ module lbfast_hip
     real(c_double) :: fx, fy, fz, visc1, visc2, rho_r, rho_b, tau_diff, sigma, width, wettab_r
     real(c_double) :: tau1, omega, tau2, beta, kapp, sharp_c, store_scale(10)
     real(c_double) :: a_rep, q_th, cos_opp_t, reserved_d
+    ! open boundaries; faces x(1) x(2) y(1) y(2) z(1) z(2)
+    integer(c_int32_t) :: openbc, openbc_type(6), reserved_i2
+    real(c_double) :: openbc_press(6), openbc_u(6), openbc_v(6), openbc_w(6)
   end type
 
   interface

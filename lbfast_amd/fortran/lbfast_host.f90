@@ -178,6 +178,13 @@ program lbfast_host
   p%visc1 = visc1; p%visc2 = visc2; p%rho_r = rho_r; p%rho_b = rho_b
   p%tau_diff = tau_diff; p%sigma = sigma; p%width = width; p%wettab_r = wettab_r
   p%a_rep = A_rep; p%q_th = q_th; p%cos_opp_t = cosOppT
+  ! open boundaries (boundary_cds_module.f90:57-1664): faces x(1) x(2) y(1) y(2) z(1) z(2)
+  p%openbc = openbc
+  p%openbc_type = [openbc_type_x, openbc_type_y, openbc_type_z]
+  p%openbc_press = [openbc_press_x, openbc_press_y, openbc_press_z]
+  p%openbc_u = [openbc_u_x, openbc_u_y, openbc_u_z]
+  p%openbc_v = [openbc_v_x, openbc_v_y, openbc_v_z]
+  p%openbc_w = [openbc_w_x, openbc_w_y, openbc_w_z]
   rc = lbf_params_derive(p)
   call check(rc, 'lbf_params_derive')
   two = p%twocomponent /= 0
@@ -187,6 +194,7 @@ program lbfast_host
     ' repulsive pbc', p%densratio, p%interface_incomp, p%internal_obstacles, p%buoyancy_forcing, &
     p%bounce_back, p%smagorinsky, p%repulsive_flux, p%pbc(1) * 100 + p%pbc(2) * 10 + p%pbc(3)
   write (6, '(a,i2)') 'lbfast_host: bcphiflux', p%bcphiflux
+  write (6, '(a,i2,a,6i2)') 'lbfast_host: openbc', p%openbc, '  types', p%openbc_type
   write (6, '(a,5g16.8)') 'lbfast_host: tau1 omega tau2 beta kapp', p%tau1, p%omega, p%tau2, p%beta, p%kapp
   flush (6)
 

@@ -36,6 +36,9 @@ class LbfError(RuntimeError):
     pass
 
 
+ABI_VERSION = 2  # LBF_ABI_VERSION of include/lbfast_hip.h (lbf_params with the open-boundary block)
+
+
 def load(path: str = LIB_PATH):
     """Load the HIP library (once).  Raises if it is not built."""
     global _lib
@@ -45,6 +48,10 @@ def load(path: str = LIB_PATH):
         raise LbfError(f"{path} is missing: build it with `python -m lbfast_amd.build` "
                        "(there is no CPU fallback)")
     L = ctypes.CDLL(path)
+    L.lbf_abi_version.restype = ctypes.c_int32
+    if L.lbf_abi_version() != ABI_VERSION:
+        raise LbfError(f"{path} has ABI version {L.lbf_abi_version()}, these bindings expect {ABI_VERSION}: "
+                       "rebuild it with `python -m lbfast_amd.build`")
     P = ctypes.POINTER(LbfParams)
     vp, ip8 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int8)
     L.lbf_abi_version.restype = ctypes.c_int32

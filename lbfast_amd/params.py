@@ -34,13 +34,17 @@ HFIELD_NAMES = ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
 STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_DIV_THETA_N = 1, 2, 3
 STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY = 4, 5, 6, 7
 STAGE_THINFILM_SCAN, STAGE_REPULSIVE_FLUX, STAGE_PHI_LAGRANGE = 8, 9, 10
+STAGE_OPEN_BC_PHI, STAGE_OPEN_BC_H = 11, 12
 # integrator_module.f90:278-524 (one FLIP half step)
 # integrator_module.f90:266-545; the repulsive stages, the phase-mass
-# correction and LB_int_boundary are no-ops unless REPULSIVE_FLUX / BCPHIFLUX /
-# INTERNAL_OBSTACLES without BOUNCE_BACK (bcs_mesoscopic_phifields runs
-# PHI_int_boundary, then the BCPHIFLUX correction: boundary_cds_module.f90:1347-1887)
-STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_PHI_LAGRANGE, STAGE_DIV_THETA_N, STAGE_THINFILM_SCAN,
-              STAGE_REPULSIVE_FLUX, STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI, STAGE_LB_INT_BOUNDARY)
+# correction, the open faces and LB_int_boundary are no-ops unless
+# REPULSIVE_FLUX / BCPHIFLUX / openbc / INTERNAL_OBSTACLES without BOUNCE_BACK
+# (bcs_mesoscopic_phifields runs PHI_int_boundary, the open faces' phase field,
+# then the BCPHIFLUX correction: boundary_cds_module.f90:1347-1887;
+# bcs_mesoscopic_hfields runs LB_int_boundary, then the open faces: 49-1311)
+STEP_ORDER = (STAGE_NORM, STAGE_PHI_INT_BOUNDARY, STAGE_OPEN_BC_PHI, STAGE_PHI_LAGRANGE, STAGE_DIV_THETA_N,
+              STAGE_THINFILM_SCAN, STAGE_REPULSIVE_FLUX, STAGE_MOMENTS, STAGE_FUSED, STAGE_UPDATE_PHI,
+              STAGE_LB_INT_BOUNDARY, STAGE_OPEN_BC_H)
 
 
 class LbfParams(ctypes.Structure):
@@ -62,6 +66,10 @@ class LbfParams(ctypes.Structure):
         ("store_scale", ctypes.c_double * 10),
         ("a_rep", ctypes.c_double), ("q_th", ctypes.c_double), ("cos_opp_t", ctypes.c_double),
         ("reserved_d", ctypes.c_double),
+        # open boundaries; faces x(1) x(2) y(1) y(2) z(1) z(2)
+        ("openbc", ctypes.c_int32), ("openbc_type", ctypes.c_int32 * 6), ("reserved_i2", ctypes.c_int32),
+        ("openbc_press", ctypes.c_double * 6), ("openbc_u", ctypes.c_double * 6),
+        ("openbc_v", ctypes.c_double * 6), ("openbc_w", ctypes.c_double * 6),
     ]
 
     @property
@@ -101,9 +109,12 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
                 fx=0.0, fy=0.0, fz=0.0, visc1=0.1, visc2=0.1, rho_r=1.0, rho_b=1.0,
                 tau_diff=0.1, sigma=0.0, width=4.0, wettab_r=90.0,
                 interface_incomp=None, buoyancy_forcing=False, bounce_back=False, smagorinsky=False,
-                repulsive_flux=False, a_rep=0.0, q_th=0.125, cos_opp_t=-0.8, bcphiflux=False) -> LbfParams:
+                repulsive_flux=False, a_rep=0.0, q_th=0.125, cos_opp_t=-0.8, bcphiflux=False,
+                openbc=False, openbc_type=(0,) * 6, openbc_press=(0.0,) * 6, openbc_u=(0.0,) * 6,
+                openbc_v=(0.0,) * 6, openbc_w=(0.0,) * 6) -> LbfParams:
     """Build a parameter block. ``interface_incomp`` defaults to ``densratio``
-    because defines.h:72-74 defines INTERFACE_INCOMP whenever DENSRATIO is set."""
+    because defines.h:72-74 defines INTERFACE_INCOMP whenever DENSRATIO is set.
+    The open-boundary sequences are per face: x(1), x(2), y(1), y(2), z(1), z(2)."""
     p = LbfParams()
     p.nx, p.ny, p.nz = n
     p.lattice = lattice
@@ -128,6 +139,11 @@ def make_params(n=(64, 64, 64), lattice=27, highorder=True, twocomponent=False,
     p.repulsive_flux = int(bool(repulsive_flux) and p.twocomponent)
     p.a_rep, p.q_th, p.cos_opp_t = a_rep, q_th, cos_opp_t
     p.bcphiflux = int(bool(bcphiflux))
+    p.openbc = int(bool(openbc))
+    for f in range(6):
+        p.openbc_type[f] = int(openbc_type[f])
+        p.openbc_press[f], p.openbc_u[f] = openbc_press[f], openbc_u[f]
+        p.openbc_v[f], p.openbc_w[f] = openbc_v[f], openbc_w[f]
     return derive(p)
 
 
@@ -186,4 +202,16 @@ def params_from_namelist(text: str, lattice=27, highorder=True, twocomponent=Tru
         tau_diff=fl.get("tau_diff", 0.1), sigma=fl.get("sigma", 0.0), width=fl.get("width", 4.0),
         wettab_r=fl.get("wettab_r", 90.0), bounce_back=bounce_back, smagorinsky=smagorinsky,
         repulsive_flux=repulsive_flux, a_rep=fl.get("a_rep", 0.0), q_th=fl.get("q_th", 0.125),
-        cos_opp_t=fl.get("cosoppt", -0.8))
+        cos_opp_t=fl.get("cosoppt", -0.8), openbc=bool(sim.get("openbc", 0)),
+        **{f"openbc_{k}": _faces(sim, k) for k in ("type", "press", "u", "v", "w")})
+
+
+def _faces(sim: dict, k: str) -> tuple:
+    """openbc_<k>_x(1:2), _y(1:2), _z(1:2) of a deck as six values per face,
+    x(1) x(2) y(1) y(2) z(1) z(2); an absent value is 0 (the declared defaults)."""
+    out = []
+    for ax in "xyz":
+        v = sim.get(f"openbc_{k}_{ax}", [0, 0])
+        v = list(v) if isinstance(v, (list, tuple)) else [v]  # a scalar sets element (1) only, as in Fortran
+        out += [(int if k == "type" else float)(x) for x in (v + [0, 0])[:2]]
+    return tuple(out)

@@ -210,7 +210,7 @@ struct Sim : SimBase {
   std::vector<R> pop;  // post-collision populations of the last fused stage
   int cur = 0;         // index of the S buffer
   int stepno = 0;
-  bool two, dens, incomp, obst, ho, bb, smag, rep, lagr;
+  bool two, dens, incomp, obst, ho, bb, smag, rep, lagr, obc;
   double phi_sum_ini = 0;  // BCPHIFLUX: global_phi_sum_ini
   R Minv[27][27];
 
@@ -238,6 +238,7 @@ struct Sim : SimBase {
     smag = p.smagorinsky != 0;
     rep = two && p.repulsive_flux != 0;
     lagr = two && obst && p.bcphiflux != 0;
+    obc = obst && p.openbc != 0;
     for (int k = 0; k < 27; ++k)
       for (int l = 0; l < 27; ++l) Minv[k][l] = (R)minv_d3q27(k, l);
   }
@@ -355,6 +356,12 @@ struct Sim : SimBase {
   }
 
   // ---- PHI_int_boundary (lb_cuda_boundary_module.f90:274-454) -------------
+  // Past a non-periodic face isfluid reads 1, so where two walls meet the
+  // donor can be a node of the upper halo (the lower one is skipped, boundary:
+  // 395).  The reference writes phi and the normals there nowhere: its
+  // initial conditions fill the interior only (init_conditions_module.f90:
+  // 60-90) and the halo exchanges copy periodic images only; here those halo
+  // nodes keep the zero they are allocated with, so such a ghost becomes 0.
   void phi_int_boundary(std::vector<T>& ph) {
     const R eps_norm = (R)1.0e-14, huge_cot = (R)1.0e14;
     const R inv_sqrt2 = (R)0.70710678118654752440, inv_sqrt3 = (R)0.57735026918962576450;
@@ -843,6 +850,164 @@ struct Sim : SimBase {
         }
   }
 
+  // ---- open boundaries (boundary_cds_module.f90:57-1311, 1353-1664) ------------------
+  // Face f = 2*axis + side (x(1) x(2) y(1) y(2) z(1) z(2)) of each non-periodic
+  // axis; the loops over the face run over the interior with the face's edges
+  // skipped (gj > 1 .and. gj < ly, ...), in the reference's face order.
+  template <class Fn>
+  void face_loop(int f, int layer_lo, int layer_hi, Fn fn) const {
+    const int ax = f >> 1, side = f & 1;
+    const int n3[3] = {nx, ny, nz};
+    if (p.pbc[ax]) return;
+    const int b = ax == 0 ? 1 : 0, c = ax == 2 ? 1 : 2;  // the two other axes
+    const int lay = side ? layer_hi : layer_lo;
+    for (int q = 2; q <= n3[c] - 1; ++q)
+      for (int r = 2; r <= n3[b] - 1; ++r) {
+        int ijk[3];
+        ijk[ax] = lay; ijk[b] = r; ijk[c] = q;
+        fn(ijk[0], ijk[1], ijk[2]);
+      }
+  }
+  // the HIGHORDER terms of the regularised population of link l (the E_k and
+  // N_k of fused:668-702 with Minv_mrt(10..26, l)), as the open-face loops write them
+  R ho_feq(int l, R u, R v, R w) const {
+    return Minv[10][l] * u * u * v + Minv[11][l] * u * u * w + Minv[12][l] * u * v * v +
+           Minv[13][l] * u * w * w + Minv[14][l] * v * w * w + Minv[15][l] * v * v * w +
+           Minv[16][l] * u * v * w + Minv[17][l] * u * u * v * v + Minv[18][l] * u * u * w * w +
+           Minv[19][l] * v * v * w * w + Minv[20][l] * u * v * w * w + Minv[21][l] * u * v * v * w +
+           Minv[22][l] * u * u * v * w + Minv[23][l] * u * u * v * w * w + Minv[24][l] * u * u * v * v * w +
+           Minv[25][l] * u * v * v * w * w + Minv[26][l] * u * u * v * v * w * w;
+  }
+  R ho_fneq(int l, R pxx, R pyy, R pzz, R pxy, R pxz, R pyz, R u, R v, R w) const {
+    const R TWO = 2, THREE = 3, FOUR = 4, SIX = 6, SEVEN = 7, TEN = 10;
+    return Minv[10][l] * (TWO * pxy * u + pxx * v) + Minv[11][l] * (TWO * pxz * u + pxx * w) +
+           Minv[12][l] * (pyy * u + TWO * pxy * v) + Minv[13][l] * (pzz * u + TWO * pxz * w) +
+           Minv[14][l] * (pzz * v + TWO * pyz * w) + Minv[15][l] * (TWO * pyz * v + pyy * w) +
+           Minv[16][l] * (pyz * u + pxz * v + pxy * w) +
+           Minv[17][l] * (pyy * u * u + TWO * v * (THREE * pxy * u + pxx * v)) +
+           Minv[18][l] * (pzz * u * u + TWO * w * (THREE * pxz * u + pxx * w)) +
+           Minv[19][l] * (pzz * v * v + TWO * w * (THREE * pyz * v + pyy * w)) +
+           Minv[20][l] * (pzz * u * v + w * (THREE * pyz * u + THREE * pxz * v + TWO * pxy * w)) +
+           Minv[21][l] * (TWO * pyz * u * v + pxz * v * v + TWO * pyy * u * w + FOUR * pxy * v * w) +
+           Minv[22][l] * (pyz * u * u + TWO * pxz * u * v + FOUR * pxy * u * w + TWO * pxx * v * w) +
+           Minv[23][l] * (pzz * u * u * v + THREE * w * (pyz * u * u + TWO * pxz * u * v + TWO * pxy * u * w + pxx * v * w)) +
+           Minv[24][l] * (TWO * u * v * (pyz * u + pxz * v) + TWO * pyy * u * u * w + v * (TEN * pxy * u + THREE * pxx * v) * w) +
+           Minv[25][l] * (pzz * u * v * v + THREE * w * (TWO * pyz * u * v + pxz * v * v + pyy * u * w + TWO * pxy * v * w)) +
+           Minv[26][l] * (pzz * u * u * v * v +
+                          w * (SIX * u * v * (pyz * u + pxz * v) + THREE * pyy * u * u * w + TWO * v * (SEVEN * pxy * u + TWO * pxx * v) * w));
+  }
+  // bcs_mesoscopic_phifields (1353-1664): the wall node takes phi of its
+  // neighbour inside the domain
+  void open_phi(std::vector<T>& ph) {
+    for (int f = 0; f < 6; ++f) {
+      const int ax = f >> 1, d = (f & 1) ? -1 : 1;
+      face_loop(f, 1, ax == 0 ? nx : (ax == 1 ? ny : nz), [&](int i, int j, int k) {
+        ph[I(i, j, k)] = ph[I(i + (ax == 0) * d, j + (ax == 1) * d, k + (ax == 2) * d)];
+      });
+    }
+    halo(ph);
+  }
+  // bcs_mesoscopic_hfields (57-1311), after LB_int_boundary: on the first
+  // fluid layer, the links from solid nodes lose the closure's population
+  // built from the node's P and Pi - cs^2 P - u u and gain the regularised
+  // population of the face state (presstmp, utmp, vtmp, wtmp by openbc_type)
+  void open_h(const std::vector<T>* hin, std::vector<T>* hout, const std::vector<T>& ph) {
+    const R cssq = (R)(1.0 / 3.0), invcssq = (R)3, HALF = (R)0.5, ONE = 1, TWO = 2;
+    for (int f = 0; f < 6; ++f) {
+      const int ax = f >> 1;
+      const int nA = ax == 0 ? nx : (ax == 1 ? ny : nz);
+      face_loop(f, 2, nA - 1, [&](int i, int j, int k) {
+        const size_t n = I(i, j, k);
+        if (isf[n] != -1) return;
+        R press = ld(hin[0], n), u = ld(hin[1], n), v = ld(hin[2], n), w = ld(hin[3], n);
+        R pxx = ld(hin[4], n), pyy = ld(hin[5], n), pzz = ld(hin[6], n);
+        R pxy = ld(hin[7], n), pxz = ld(hin[8], n), pyz = ld(hin[9], n);
+        R op = ld(hout[0], n), ou = ld(hout[1], n), ov = ld(hout[2], n), ow = ld(hout[3], n);
+        R oxx = ld(hout[4], n), oyy = ld(hout[5], n), ozz = ld(hout[6], n);
+        R oxy = ld(hout[7], n), oxz = ld(hout[8], n), oyz = ld(hout[9], n);
+        R presstmp = press, utmp = u, vtmp = v, wtmp = w;
+        if (p.openbc_type[f] == 1) {
+          presstmp = (R)p.openbc_press[f];
+        } else if (p.openbc_type[f] != 2) {
+          utmp = (R)p.openbc_u[f]; vtmp = (R)p.openbc_v[f]; wtmp = (R)p.openbc_w[f];
+        }
+        pxx = pxx - cssq * press - u * u;
+        pyy = pyy - cssq * press - v * v;
+        pzz = pzz - cssq * press - w * w;
+        pxy = pxy - u * v;
+        pxz = pxz - u * w;
+        pyz = pyz - v * w;
+        R om;
+        if (two) {
+          const R phl = ld(ph, n);
+          const R rhophi = dens ? (R)p.rho_r * phl + (ONE - phl) * (R)p.rho_b : ONE;
+          const R visc = ((R)p.rho_r * (R)p.visc1 * phl + ((R)1.0 - phl) * (R)p.visc2 * (R)p.rho_b) / rhophi;
+          const R tau = (visc * invcssq + HALF);
+          om = ONE / tau;
+        } else {
+          om = (R)p.omega;
+        }
+        const R uu = HALF * (utmp * utmp + vtmp * vtmp + wtmp * wtmp) * invcssq;
+        for (int l = 1; l < L.nl; ++l) {
+          const int lo = OPP[l];
+          if (isf[I(i + EX[lo], j + EY[lo], k + EZ[lo])] != 0) continue;
+          const R ex = (R)EX[l], ey = (R)EY[l], ez = (R)EZ[l], wl = (R)L.w[l];
+          R feq = wl * (press);
+          R fneq1 = wl * (HALF / (cssq * cssq)) *
+                    ((ex * ex - cssq) * pxx + (ey * ey - cssq) * pyy + (ez * ez - cssq) * pzz +
+                     TWO * (ex * ey) * pxy + TWO * (ex * ez) * pxz + TWO * (ey * ez) * pyz);
+          R fpost = feq + (ONE - om) * fneq1;
+          op = op - fpost;
+          ou = ou - fpost * ex; ov = ov - fpost * ey; ow = ow - fpost * ez;
+          oxx = oxx - fpost * ex * ex; oyy = oyy - fpost * ey * ey; ozz = ozz - fpost * ez * ez;
+          oxy = oxy - fpost * ex * ey; oxz = oxz - fpost * ex * ez; oyz = oyz - fpost * ey * ez;
+          const R udotc = (utmp * ex + vtmp * ey + wtmp * ez) * invcssq;
+          feq = wl * (presstmp + udotc + HALF * udotc * udotc - uu);
+          fneq1 = wl * (HALF / (cssq * cssq)) *
+                  ((ex * ex - cssq) * pxx + (ey * ey - cssq) * pyy + (ez * ez - cssq) * pzz +
+                   TWO * (ex * ey) * pxy + TWO * (ex * ez) * pxz + TWO * (ey * ez) * pyz);
+          if (ho) {
+            feq = feq + ho_feq(l, utmp, vtmp, wtmp);
+            fneq1 = fneq1 + ho_fneq(l, pxx, pyy, pzz, pxy, pxz, pyz, utmp, vtmp, wtmp);
+          }
+          fpost = feq + (ONE - om) * fneq1;
+          op = op + fpost;
+          ou = ou + fpost * ex; ov = ov + fpost * ey; ow = ow + fpost * ez;
+          oxx = oxx + fpost * (ex * ex - cssq); oyy = oyy + fpost * (ey * ey - cssq);
+          ozz = ozz + fpost * (ez * ez - cssq);
+          oxy = oxy + fpost * ex * ey; oxz = oxz + fpost * ex * ez; oyz = oyz + fpost * ey * ez;
+        }
+        hout[0][n] = st(op); hout[1][n] = st(ou); hout[2][n] = st(ov); hout[3][n] = st(ow);
+        hout[4][n] = st(oxx); hout[5][n] = st(oyy); hout[6][n] = st(ozz);
+        hout[7][n] = st(oxy); hout[8][n] = st(oxz); hout[9][n] = st(oyz);
+      });
+    }
+  }
+  // BCPHIFLUX with open boundaries (1667-1862): the phase carried through the
+  // first fluid layer of each open face, + u_a phi at the low face and
+  // - u_a phi at the high one, u the velocity along the face's axis in S;
+  // every node of the layer with isfluid -1, edges included
+  double open_flux(const std::vector<T>* hs, const std::vector<T>& ph) const {
+    double chg = 0;
+    const int n3[3] = {nx, ny, nz};
+    for (int f = 0; f < 6; ++f) {
+      const int ax = f >> 1, side = f & 1;
+      if (p.pbc[ax]) continue;
+      const int lay = side ? n3[ax] - 1 : 2;
+      for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+          for (int i = 1; i <= nx; ++i) {
+            const int ijk[3] = {i, j, k};
+            if (ijk[ax] != lay) continue;
+            const size_t n = I(i, j, k);
+            if (isf[n] != -1) continue;
+            const double ua = (double)hs[1 + ax][n], phl = (double)ph[n];
+            chg = side ? chg - ua * phl : chg + ua * phl;
+          }
+    }
+    return chg;
+  }
+
   // ---- BCPHIFLUX phase-mass correction ------------------------------------------
   // phi_sum_count_kernel / apply_lagrangian_phi_kernel (lb_cuda_boundary_module.f90:
   // 461-566) as boundary_cds_module.f90:1864-1884 drives them, without open
@@ -870,7 +1035,10 @@ struct Sim : SimBase {
     double sum;
     long cnt;
     phi_sums(ph, sum, cnt);
-    const double corr = cnt > 0 ? (phi_sum_ini - sum) / (double)(float)cnt : 0.0;
+    // dphi = (global_phi_sum_ini + global_phi_change) - global_phi_sum, the
+    // change being this step's flux through the open faces (zero without them)
+    const double chg = obc ? open_flux(h[cur], ph) : 0.0;
+    const double corr = cnt > 0 ? ((phi_sum_ini + chg) - sum) / (double)(float)cnt : 0.0;
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) {
@@ -902,6 +1070,8 @@ struct Sim : SimBase {
       case LBF_STAGE_THINFILM_SCAN: if (!rep) return 0; thinfilm_scan(pS); return 0;
       case LBF_STAGE_REPULSIVE_FLUX: if (!rep) return 0; repulsive_flux(pS); return 0;
       case LBF_STAGE_PHI_LAGRANGE: if (!lagr) return 0; phi_lagrange(pS); return 0;
+      case LBF_STAGE_OPEN_BC_PHI: if (!two || !obc) return 0; open_phi(pS); return 0;
+      case LBF_STAGE_OPEN_BC_H: if (!obc) return 0; open_h(S, O, pS); return 0;
       default: err = "unknown stage"; return LBF_EINVAL;
     }
   }
@@ -912,6 +1082,7 @@ struct Sim : SimBase {
       begin_step();
       stage(LBF_STAGE_NORM);
       stage(LBF_STAGE_PHI_INT_BOUNDARY);
+      stage(LBF_STAGE_OPEN_BC_PHI);
       stage(LBF_STAGE_PHI_LAGRANGE);
       stage(LBF_STAGE_DIV_THETA_N);
       stage(LBF_STAGE_THINFILM_SCAN);
@@ -920,6 +1091,7 @@ struct Sim : SimBase {
       stage(LBF_STAGE_FUSED);
       stage(LBF_STAGE_UPDATE_PHI);
       stage(LBF_STAGE_LB_INT_BOUNDARY);
+      stage(LBF_STAGE_OPEN_BC_H);
       end_step();
     }
     return 0;
@@ -935,6 +1107,7 @@ struct Sim : SimBase {
       phi_sums(phi[cur], phi_sum_ini, cnt);
     }
     if (two && obst) phi_int_boundary(phi[cur]);  // bcs_phi before the loop
+    if (two && obc) open_phi(phi[cur]);
     if (lagr) phi_lagrange(phi[cur]);
     if (two) halo(phi[cur]);
     for (int f = 0; f < 10; ++f) halo(h[cur][f]);

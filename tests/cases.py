@@ -70,11 +70,15 @@ def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
         f = setups.droplet(n, R, 3.0, [n[0] / 2, n[1] / 2, n[2] / 2], 0.01, 1.0, 1.0, densratio=False,
                            noise=1e-4, seed=seed)
         return p, f, None
-    if name in ("chan27h", "chan19"):  # walls in y, spheres, two layers, body force
+    if name in ("chan27h", "chan19", "duct27h"):
+        # walls in y (duct27h: in x and y, so that walls meet along edges),
+        # spheres, two layers, body force
         lat = 19 if name == "chan19" else 27
+        pbc = (0, 0, 1) if name == "duct27h" else (1, 0, 1)
         p = make_params(n, lattice=lat, highorder=(lat == 27), twocomponent=True, densratio=True,
-                        internal_obstacles=True, pbc=(1, 0, 1), visc1=0.05, visc2=0.1, rho_r=2.0, rho_b=1.0,
-                        tau_diff=0.1, sigma=0.005, width=4.0, fx=1e-5, precision=precision)
+                        internal_obstacles=True, pbc=pbc, visc1=0.05, visc2=0.1, rho_r=2.0, rho_b=1.0,
+                        tau_diff=0.1, sigma=0.005, width=4.0, fx=0.0 if name == "duct27h" else 1e-5,
+                        fz=1e-5 if name == "duct27h" else 0.0, precision=precision)
         raw = setups.random_spheres(n, fill=0.15, rmin=2.0, rmax=4.0, seed=seed, margin=2)
         f = setups.two_layer_channel(n, raw, frac=0.4, width=4.0, rho_r=2.0, rho_b=1.0, axis=1)
         return p, f, raw
@@ -91,6 +95,38 @@ def case(name: str, n=(32, 32, 32), precision=PREC_FP64, seed=1):
         uvw[:, :, :, -1] = 0.0
         f.update(setups._moments(f["P"], *uvw))
         return p, f, np.ones((n[2], n[1], n[0]), dtype=np.int8)
+    if name in ("duct27h_open", "duct19_open"):
+        # open boundaries, one component: a duct along x with spheres, walls
+        # in y and z (type-0 faces at rest); 27h: pressure inlet and outlet,
+        # 19: velocity inlet, local (type 2) outlet, the z(2) face moving
+        lat = 19 if name == "duct19_open" else 27
+        p = make_params(n, lattice=lat, highorder=(lat == 27), internal_obstacles=True, pbc=(0, 0, 0),
+                        visc1=0.05, precision=precision)
+        p.openbc = 1
+        if lat == 27:
+            p.openbc_type[0], p.openbc_press[0] = 1, 3e-3
+            p.openbc_type[1], p.openbc_press[1] = 1, 0.0
+        else:
+            p.openbc_type[0], p.openbc_u[0], p.openbc_v[0] = 0, 2e-3, 1e-4
+            p.openbc_type[1] = 2
+            p.openbc_type[5], p.openbc_u[5] = 0, 1e-3
+        raw = setups.random_spheres(n, fill=0.1, rmin=2.0, rmax=3.0, seed=seed, margin=3)
+        rng = np.random.default_rng(seed)
+        uvw = rng.uniform(-1e-4, 1e-4, (3, n[2], n[1], n[0]))
+        P = rng.uniform(-1e-4, 1e-4, (n[2], n[1], n[0]))
+        return p, setups._moments(P, *uvw), raw
+    if name == "chan27h_open":
+        # two components through open x faces: a velocity inlet and a
+        # pressure outlet, walls in y (type-0 faces at rest), z periodic
+        p = make_params(n, lattice=27, highorder=True, twocomponent=True, densratio=True,
+                        internal_obstacles=True, pbc=(0, 0, 1), visc1=0.05, visc2=0.1, rho_r=2.0, rho_b=1.0,
+                        tau_diff=0.1, sigma=0.005, width=4.0, precision=precision)
+        p.openbc = 1
+        p.openbc_type[0], p.openbc_u[0] = 0, 3e-3
+        p.openbc_type[1], p.openbc_press[1] = 1, 0.0
+        raw = setups.random_spheres(n, fill=0.1, rmin=2.0, rmax=3.0, seed=seed, margin=3)
+        f = setups.two_layer_channel(n, raw, frac=0.4, width=4.0, rho_r=2.0, rho_b=1.0, axis=1)
+        return p, f, raw
     if name == "pois27h_1c":  # one component with walls and obstacles
         p = make_params(n, lattice=27, highorder=True, internal_obstacles=True, pbc=(1, 0, 1), visc1=0.1,
                         fx=1e-5, precision=precision)

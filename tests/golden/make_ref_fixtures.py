@@ -15,6 +15,12 @@ defines.h preprocessor switches):
                            -> the near-wall closure's output moments
   PHI_int_boundary_kernel  lb_cuda_boundary_module.f90:218-459
                            -> the wetting ghost value of a solid node
+  open face x(1)           boundary_cds_module.f90:104-262 (the body of the
+                           x(1) loop of bcs_mesoscopic_hfields; the other five
+                           faces' bodies are the same statements with their
+                           own openbc_*(n))
+                           -> the node's output moments after the open-face
+                              correction, for openbc_type 0, 1 and 2
 
 Each kernel runs under several switch sets.  The lattice tables are the
 reference's own: ex, ey, ez, opp, p and the constants come from evaluating
@@ -656,6 +662,60 @@ def gen_phibnd(ref, name, defines, wettab, rng, n=96):
     return {"phi": phi, "normal": nrm, "modgrad": mg, "neighbours": nb, "phi_out": out}
 
 
+OPENH_CASES = {
+    "openh_d3q27h_1c_t1": (dict(LATTICE=27, HIGHORDER=1, INTERNAL_OBSTACLES=1), 1),
+    "openh_d3q27h_1c_t0": (dict(LATTICE=27, HIGHORDER=1, INTERNAL_OBSTACLES=1), 0),
+    "openh_d3q27h_2c_dens_t2": (dict(LATTICE=27, HIGHORDER=1, TWOCOMPONENT=1, DENSRATIO=1, INTERNAL_OBSTACLES=1), 2),
+    "openh_d3q27h_2c_dens_t0": (dict(LATTICE=27, HIGHORDER=1, TWOCOMPONENT=1, DENSRATIO=1, INTERNAL_OBSTACLES=1), 0),
+    "openh_d3q27_2c_t1": (dict(LATTICE=27, TWOCOMPONENT=1, INTERNAL_OBSTACLES=1), 1),
+    "openh_d3q19_1c_t0": (dict(LATTICE=19, INTERNAL_OBSTACLES=1), 0),
+}
+
+
+def open_face_body(ref):
+    """The per-node statements of the first open face of
+    bcs_mesoscopic_hfields: from the load of hfields_in to the store of the
+    tenth output moment."""
+    lines = read(ref, "boundary_cds_module.f90")
+    a = next(i for i, l in enumerate(lines) if "press=real(hfields_in(ii,jj,kk,1,myblock)" in l.replace(" ", ""))
+    b = next(i for i in range(a, len(lines)) if "hfields_out(ii,jj,kk,10,myblock)=real(opyz" in lines[i].replace(" ", ""))
+    return lines[a:b + 1]
+
+
+def gen_openh(ref, name, defines, otype, rng, n=64):
+    tabs = module_tables(ref, defines)
+    body = parse(statements(preprocess(open_face_body(ref), defines)))[0]
+    face = dict(press=float(rng.uniform(-5e-3, 5e-3)), u=float(rng.uniform(-0.05, 0.05)),
+                v=float(rng.uniform(-0.05, 0.05)), w=float(rng.uniform(-0.05, 0.05)))
+    prm = derived(PARAMS_2C)
+    prm["openbc_type_x"] = lambda k: [otype, 2][int(k) - 1]
+    prm["openbc_press_x"] = lambda k: [face["press"], 0.0][int(k) - 1]
+    prm["openbc_u_x"] = lambda k: [face["u"], 0.0][int(k) - 1]
+    prm["openbc_v_x"] = lambda k: [face["v"], 0.0][int(k) - 1]
+    prm["openbc_w_x"] = lambda k: [face["w"], 0.0][int(k) - 1]
+    mom = _mom_inputs(rng, n)
+    oin = _mom_inputs(rng, n)
+    phi = rng.uniform(0.0, 1.0, n)
+    # the wall (x - 1) is solid, the other neighbours solid or fluid; the node
+    # itself is a near-wall fluid node (-1)
+    nb = (rng.uniform(size=(27, n)) >= 0.35).astype(np.int8)
+    for q, o in enumerate(OFFS):
+        if o[0] == -1:
+            nb[q] = 0
+    nb[13] = -1
+    out = np.zeros((10, n))
+    for s in range(n):
+        fields = {"hfields_in": {c + 1: {(0, 0, 0): float(mom[c, s])} for c in range(10)},
+                  "hfields_out": {c + 1: {(0, 0, 0): float(oin[c, s])} for c in range(10)},
+                  "phifields_s": {1: {(0, 0, 0): float(phi[s])}}}
+        isf = {o: int(nb[q, s]) for q, o in enumerate(OFFS)}
+        store, _, _ = run_node(body, defines, tabs, prm, fields, isf)
+        for c in range(10):
+            out[c, s] = store[("hfields_out", c + 1, (0, 0, 0))]
+    return {"moments": mom, "moments_o": oin, "phi": phi, "neighbours": nb, "moments_o_out": out,
+            "face": np.array([otype, face["press"], face["u"], face["v"], face["w"]])}
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     rng = np.random.Generator(np.random.MT19937(20261203))
@@ -674,6 +734,8 @@ def main():
         put(case, "lb_int_boundary", d, {}, gen_lbint(ref, case, d, rng))
     for case, (d, wt) in PHIBND_CASES.items():
         put(case, "phi_int_boundary", d, {"wettab_r": wt, "wettab_b": wt}, gen_phibnd(ref, case, d, wt, rng))
+    for case, (d, t) in OPENH_CASES.items():
+        put(case, "open_h", d, {}, gen_openh(ref, case, d, t, rng))
     arrays["meta"] = np.frombuffer(json.dumps(meta, sort_keys=True).encode(), dtype=np.uint8)
     out = os.path.join(HERE, "ref_kernels.npz")
     np.savez_compressed(out, **arrays)

@@ -35,7 +35,10 @@ def test_params_struct_layout_matches_header():
     assert (LbfParams.fx.offset, LbfParams.tau1.offset, LbfParams.store_scale.offset) == (72, 160, 208)
     assert (LbfParams.bounce_back.offset, LbfParams.smagorinsky.offset, LbfParams.repulsive_flux.offset) == (56, 60, 64)
     assert (LbfParams.a_rep.offset, LbfParams.q_th.offset, LbfParams.cos_opp_t.offset) == (288, 296, 304)
-    assert ctypes.sizeof(LbfParams) == 320
+    assert (LbfParams.openbc.offset, LbfParams.openbc_type.offset, LbfParams.reserved_i2.offset) == (320, 324, 348)
+    assert (LbfParams.openbc_press.offset, LbfParams.openbc_u.offset, LbfParams.openbc_v.offset,
+            LbfParams.openbc_w.offset) == (352, 400, 448, 496)
+    assert ctypes.sizeof(LbfParams) == 544
     L = hip.load()
     p = LbfParams()
     L.lbf_params_default(ctypes.byref(p))

@@ -97,6 +97,10 @@ program layout
   write (6, '(a,i0)') 'store_scale ', transfer(c_loc(p%store_scale), b) - b
   write (6, '(a,i0)') 'a_rep ', transfer(c_loc(p%a_rep), b) - b
   write (6, '(a,i0)') 'reserved_d ', transfer(c_loc(p%reserved_d), b) - b
+  write (6, '(a,i0)') 'openbc ', transfer(c_loc(p%openbc), b) - b
+  write (6, '(a,i0)') 'openbc_type ', transfer(c_loc(p%openbc_type), b) - b
+  write (6, '(a,i0)') 'openbc_press ', transfer(c_loc(p%openbc_press), b) - b
+  write (6, '(a,i0)') 'openbc_w ', transfer(c_loc(p%openbc_w), b) - b
 end program
 """
 
@@ -113,6 +117,6 @@ def test_fortran_params_layout_matches_c(tmp_path):
                     f"-L{libdir}", "-llbfast_hip", f"-Wl,-rpath,{libdir}"], check=True)
     out = subprocess.run([str(tmp_path / "layout")], capture_output=True, text=True, check=True).stdout
     got = {k: int(v) for k, v in (line.split() for line in out.strip().splitlines())}
-    assert got.pop("size") == ctypes.sizeof(LbfParams) == 320
+    assert got.pop("size") == ctypes.sizeof(LbfParams) == 544
     for k, off in got.items():
         assert getattr(LbfParams, k).offset == off, k

@@ -107,7 +107,7 @@ def test_two_component_fast(name, prec):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("name", ["chan27h", "chan19", "pois27h_1c"])
+@pytest.mark.parametrize("name", ["chan27h", "chan19", "pois27h_1c", "duct27h"])
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
 def test_obstacles_fast(name, prec):
     res, bad = run_pair(name, prec, (40, 24, 32), 10)
@@ -169,6 +169,23 @@ def test_smagorinsky(name, prec, path):
 def test_bcphiflux(name, prec, path):
     """BCPHIFLUX: the global phase-mass correction after the wetting update."""
     res, bad = run_pair(name, prec, (40, 24, 32), 8, path=path)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["duct27h_open", "duct19_open", "chan27h_open", "chan27h_open_bcphi"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32, PREC_MIXED])
+def test_open_faces(name, prec):
+    """Open boundaries (bcs_mesoscopic_hfields / _phifields): pressure,
+    velocity and local faces on all three axes, one and two components, and
+    BCPHIFLUX with the open faces' phase flux.  lbf_step runs the stage
+    order for them."""
+    res, bad = run_pair(name, prec, (40, 24, 32), 10)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["duct27h_open", "chan27h_open_bcphi"])
+def test_open_faces_staged(name):
+    res, bad = run_pair(name, PREC_FP64, (33, 20, 18), 10, path="staged")
     assert not bad, bad
 
 

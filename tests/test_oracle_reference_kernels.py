@@ -21,6 +21,9 @@ numbers only):
   phibnd_*    PHI_int_boundary_kernel (lb_cuda_boundary_module.f90:274-454):
               the wetting ghost value of a solid node next to plane walls,
               edges and scattered solids, for contact angles 60, 90, 130
+  openh_*     the open-face correction of bcs_mesoscopic_hfields
+              (boundary_cds_module.f90:104-262, the x(1) face): the output
+              moments of a first-layer node for openbc_type 0, 1 and 2
 
 The oracle runs the same inputs through its own stages (FP64) and must agree
 to round-off.  Also here: the HIGHORDER equilibrium by mathematics -- with
@@ -40,7 +43,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 _DATA = np.load(os.path.join(GOLD, "ref_kernels.npz"))
 META = json.loads(_DATA["meta"].tobytes().decode())
 
-STAGE_MOMENTS, STAGE_FUSED, STAGE_LB_INT, STAGE_PHI_BND = 4, 5, 7, 2
+STAGE_MOMENTS, STAGE_FUSED, STAGE_LB_INT, STAGE_PHI_BND, STAGE_OPEN_H = 4, 5, 7, 2, 12
 OFFS = [(a, b, c) for c in (-1, 0, 1) for b in (-1, 0, 1) for a in (-1, 0, 1)]  # as the generator
 
 
@@ -167,6 +170,51 @@ def test_lb_int_boundary_matches_reference_statements(name):
     got = np.stack([got[:, z, y, x] for (x, y, z) in cen], axis=1)
     err = _rel(got, c["moments_o_out"])
     assert err < 1e-13, (name, err)
+
+
+@pytest.mark.parametrize("name", sorted(k for k, m in META.items() if m["kernel"] == "open_h"))
+def test_open_face_matches_reference_statements(name):
+    """The open-face correction of bcs_mesoscopic_hfields on the x(1) face:
+    64 sample nodes on the first fluid layer (x = 1, 0-based), 3 apart in y
+    and z, inside the face's edges; y and z periodic so no other face acts
+    on them."""
+    m, c = META[name], _case(name)
+    A = 8
+    n = (4, 3 * A, 3 * A)
+    p = _params(m, n)
+    p.pbc[0], p.pbc[1], p.pbc[2] = 0, 1, 1
+    t, press, u, v, w = c["face"]
+    p.openbc = 1
+    p.openbc_type[0], p.openbc_type[1] = int(t), 2
+    p.openbc_press[0], p.openbc_u[0], p.openbc_v[0], p.openbc_w[0] = press, u, v, w
+    nb = c["neighbours"]
+    cen = [(1, 3 * b + 1, 3 * a + 1) for a in range(A) for b in range(A)]
+    assert len(cen) == nb.shape[1]
+    isf = np.ones((n[2], n[1], n[0]), dtype=np.int8)
+    S = np.zeros((10, n[2], n[1], n[0]))
+    O = np.zeros((10, n[2], n[1], n[0]))
+    phi = np.full((n[2], n[1], n[0]), 0.5)
+    for s, (x, y, z) in enumerate(cen):
+        for q, (a, b, cc) in enumerate(OFFS):
+            isf[z + cc, y + b, x + a] = nb[q, s]
+        S[:, z, y, x] = c["moments"][:, s]
+        O[:, z, y, x] = c["moments_o"][:, s]
+        phi[z, y, x] = c["phi"][s]
+    o = oracle.Oracle(p)
+    o.set_isfluid(isf)
+    for f in range(10):
+        o.set_field(1 + f, S[f])
+        o.set_field(27 + f, O[f])
+    if p.twocomponent:
+        o.set_field(11, phi)
+    o.begin_step()
+    o.stage(STAGE_OPEN_H)
+    got = np.stack([o.get_field(27 + f) for f in range(10)])
+    got = np.stack([got[:, z, y, x] for (x, y, z) in cen], axis=1)
+    err = _rel(got, c["moments_o_out"])
+    assert err < 1e-13, (name, err)
+    # the correction is not a no-op: the reference's output differs from its input
+    assert np.abs(c["moments_o_out"] - c["moments_o"]).max() > 1e-3
 
 
 @pytest.mark.parametrize("name", sorted(k for k, m in META.items() if m["kernel"] == "phi_int_boundary"))
