@@ -59,6 +59,9 @@ def header(stdout: str) -> dict:
         out[k] = v
     m = re.search(r"lbfast_host: bcphiflux\s+(\d+)", stdout)
     out["bcphiflux"] = int(m.group(1)) if m else 0
+    m = re.search(r"lbfast_host: openbc\s+(\d+)\s+types\s+(.*)$", stdout, re.M)
+    out["openbc"] = int(m.group(1)) if m else 0
+    out["openbc_type"] = tuple(int(x) for x in m.group(2).split()) if m else (0,) * 6
     m = re.search(r"tau1 omega tau2 beta kapp\s+(.*)$", stdout, re.M)
     out["tau1"], out["omega"], out["tau2"], out["beta"], out["kapp"] = (float(x) for x in m.group(1).split())
     return out

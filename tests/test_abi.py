@@ -80,6 +80,27 @@ def test_create_rejects_bad_switches():
         assert L.lbf_create(ctypes.byref(p), ctypes.byref(h)) == 1
 
 
+def test_create_rejects_bad_open_boundaries():
+    """Open faces exist only in the obstacle build (boundary_cds_module.f90:
+    49-57), run the stage sequence (no 16-bit storage) and take the types 0,
+    1 and 2; each violation is refused before the device is touched."""
+    L = hip.load()
+    cases = [
+        (dict(internal_obstacles=False), {}, b"INTERNAL_OBSTACLES"),
+        (dict(internal_obstacles=True, precision=3), {}, b"stage path"),
+        (dict(internal_obstacles=True), {"type": 3}, b"openbc_type 3"),
+        (dict(internal_obstacles=True), {"type": -1}, b"openbc_type -1"),
+    ]
+    for kw, extra, msg in cases:
+        p = make_params((16, 16, 16), pbc=(0, 1, 1), **kw)
+        p.openbc = 1
+        if "type" in extra:
+            p.openbc_type[1] = extra["type"]
+        h = ctypes.c_void_p()
+        assert L.lbf_create(ctypes.byref(p), ctypes.byref(h)) == 1 and not h.value
+        assert msg in L.lbf_global_last_error(), (kw, extra, L.lbf_global_last_error())
+
+
 @pytest.mark.parametrize("n,prec,ok", [
     ((8192, 8192, 8), 0, False),    # FP64: 10 * 8192 * 8192 * 8 B per plane = 5.4 GB
     ((16384, 8192, 2), 1, False),   # FP32: 10 * 16384 * 8192 * 4 B = 5.4 GB

@@ -58,6 +58,17 @@ def test_switch_implications(tmp_path, variant, want):
         assert h[k] == v, (k, h[k], v)
 
 
+def test_open_face_keys_reach_the_parameters(tmp_path):
+    """The reference's openbc keys (LBFAST.f90:33-37): one (low, high) pair
+    per axis, passed on in the face order x(1) x(2) y(1) y(2) z(1) z(2)."""
+    sim = {"nsteps": 0, "lx": 12, "ly": 16, "lz": 8, "lprint": False, "pbc_x": 0, "pbc_y": 0, "pbc_z": 1,
+           "openbc": 1, "openbc_type_x": [0, 2], "openbc_type_y": [1, 1], "openbc_press_y": [1e-4, 0.0]}
+    r = fhost.run("pois19_fp64", tmp_path, fhost.deck(sim, {"visc1": 0.1}))
+    _ran_or_no_device(r)
+    h = fhost.header(r.stdout)
+    assert h["openbc"] == 1 and h["openbc_type"] == (0, 2, 1, 1, 0, 0)
+
+
 def test_namelist_rejects_names_of_other_builds(tmp_path):
     # sigma belongs to &fluid only when TWOCOMPONENT is compiled (LBFAST.f90:39-59)
     text = fhost.deck({"nsteps": 0, "lx": 8, "ly": 8, "lz": 8, "lprint": False}, {"visc1": 0.1, "sigma": 0.01})

@@ -19,6 +19,8 @@
   frame of the same step (print_module.f90:2007-2660).
 * BCPHIFLUX: the host's switch reaches the library, and the phase mass of a
   two-layer channel stays nearer its start than without it.
+* Open boundaries: a deck with pressure faces drives the flow from the high
+  to the low pressure, and its frames equal a hip.Domain run bit for bit.
 """
 import os
 
@@ -245,6 +247,41 @@ def test_poiseuille_bounce_back_channel(tmp_path):
     f["w"], f["pzz"] = wz, wz * wz
     d = hip.Domain(p)
     d.load(f, isf)
+    d.step(steps)
+    got = np.stack([_f32(d.download(i)) for i in (2, 3, 4)], axis=-1)
+    d.close()
+    assert np.array_equal(fhost.read_frame(tmp_path, "vel", 1, n, 3), got)
+
+
+def test_open_faces_through_the_host(tmp_path):
+    """Open boundaries from a deck: a pressure inlet and outlet on the y
+    faces, the x walls as velocity faces at rest, z periodic, starting at
+    rest.  The flow runs from y(1) to y(2), and the frames equal those of a
+    hip.Domain run with the same parameters bit for bit."""
+    variant = "pois19_fp64"
+    n, nu, dP, steps = (12, 30, 4), 1.0 / 6.0, 1e-4, 300
+    sim = _sim(n, steps, pbc_x=0, pbc_y=0, openbc=1, openbc_type_x=[0, 0], openbc_type_y=[1, 1],
+               openbc_press_y=[dP, 0.0])
+    out = _run(variant, tmp_path, sim, {"visc1": nu, "fz": 0.0})
+    h = fhost.header(out)
+    assert h["openbc"] == 1 and h["openbc_type"] == (0, 0, 1, 1, 0, 0)
+    vel = fhost.read_frame(tmp_path, "vel", 1, n, 3).astype(np.float64)
+    v = vel[..., 1]
+    assert v[:, 2:-2, 2:-2].min() > 0.0  # from the high-pressure face y(1) to y(2)
+    # mid-channel: mirror-symmetric across x (both x faces at rest; the
+    # frames are float32, so to one float32 ulp), peaked between the walls
+    mid = v[:, n[1] // 2, :]
+    assert np.allclose(mid, mid[:, ::-1], rtol=2e-7, atol=1e-15)
+    assert np.argmax(mid[0]) in (n[0] // 2 - 1, n[0] // 2)
+    p = make_params(n, lattice=19, highorder=False, internal_obstacles=True, pbc=(0, 0, 1), visc1=nu,
+                    precision=PREC_FP64)
+    p.openbc = 1
+    p.openbc_type[2], p.openbc_type[3] = 1, 1
+    p.openbc_press[2], p.openbc_press[3] = dP, 0.0
+    isf = hip.classify_isfluid(p)
+    shape = (n[2], n[1], n[0])
+    d = hip.Domain(p)
+    d.load({k: np.zeros(shape) for k in ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")}, isf)
     d.step(steps)
     got = np.stack([_f32(d.download(i)) for i in (2, 3, 4)], axis=-1)
     d.close()
