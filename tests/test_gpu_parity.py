@@ -183,6 +183,16 @@ def test_open_faces(name, prec):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("name", ["duct27h_open_bb", "duct19_open_bb", "chan27h_open_bb", "duct27h_open_smag"])
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_open_faces_with_switches(name, prec):
+    """Open faces under BOUNCE_BACK (bcs_mesoscopic_hfields applies them after
+    the reflection, boundary_cds_module.f90:49-57) and SMAGORINSKI (the open
+    faces relax with the plain rate)."""
+    res, bad = run_pair(name, prec, (40, 24, 32), 10)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("name", ["duct27h_open", "chan27h_open_bcphi"])
 def test_open_faces_staged(name):
     res, bad = run_pair(name, PREC_FP64, (33, 20, 18), 10, path="staged")
