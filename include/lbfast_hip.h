@@ -205,7 +205,10 @@ int lbf_init_state(lbf_ctx* ctx);
 
 /* Advance nsteps full time steps with the fused fast path (stream-ordered;
  * returns after enqueueing).  Exactly nsteps: a host reproducing the
- * reference loop's pairing (integrator_module.f90:266) passes 2*ceil(n/2). */
+ * reference loop's pairing (integrator_module.f90:266) passes 2*ceil(n/2).
+ * With open faces each step adds the open-face phase copy before and the
+ * open-face correction after the fused launch; REPULSIVE_FLUX runs the
+ * stage sequence. */
 int lbf_step(lbf_ctx* ctx, int32_t nsteps);
 
 /* Stage path: the reference's per-kernel entry points, in the reference's

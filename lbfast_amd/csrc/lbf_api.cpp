@@ -38,6 +38,7 @@ struct lbf_ctx {
   void* dvt = nullptr;
   void* frc = nullptr;  // 3-field set (stage path)
   void* pre = nullptr;  // 5-field set (fast path phase-field prep)
+  void* sopen = nullptr;  // open faces: post-moments state of the fast path (10-field set)
   int8_t* isfh = nullptr;
   void* rep = nullptr;         // REPULSIVE_FLUX: locaux 3..8 (6-field set)
   int8_t* rep_mask = nullptr;  // REPULSIVE_FLUX
@@ -283,7 +284,7 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     // compiled only with INTERNAL_OBSTACLES (boundary_cds_module.f90:49-1311)
     if (!p.internal_obstacles) return fail(nullptr, LBF_EINVAL, "open boundaries require INTERNAL_OBSTACLES");
     if (p.precision == LBF_PREC_FP16S || p.precision == LBF_PREC_BF16S)
-      return fail(nullptr, LBF_EINVAL, "open boundaries run the stage path: FP64, FP32 or MIXED storage");
+      return fail(nullptr, LBF_EINVAL, "open boundaries need FP64, FP32 or MIXED storage (no 16-bit codec for the open-face correction)");
     for (int f = 0; f < 6; ++f)
       if (p.openbc_type[f] < 0 || p.openbc_type[f] > 2)
         return fail(nullptr, LBF_EINVAL, "openbc_type %d of face %d: 0 (velocity), 1 (pressure) or 2 (local)",
@@ -345,6 +346,10 @@ int lbf_create(const lbf_params* pin, lbf_ctx** out) {
     c->pre = A(5 * c->NS * c->asz);
   }
   c->frc = A(3 * c->NS * c->asz);
+  // open faces on the fast path: the fused kernel hands the open-face
+  // correction the post-moments state of the wall-adjacent nodes here (a
+  // whole set, so that it shares the moment sets' layout and offsets)
+  if (p.openbc && p.internal_obstacles) c->sopen = A(10 * c->NS * c->tsz);
   if (c->dp.rep) {
     c->rep = A(6 * c->NS * c->asz);
     c->rep_mask = (int8_t*)A(c->N);
@@ -593,6 +598,7 @@ static FusedArgs fused_args(lbf_ctx* c) {
   a.tz = c->tz;
   a.seed = (uint32_t)c->step;
   a.seedp = nullptr;
+  a.sopen = c->sopen;
   return a;
 }
 
@@ -694,26 +700,34 @@ int lbf_step(lbf_ctx* c, int32_t nsteps) {
   HIPCHK(c, hipEventRecord(c->t0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
     ++c->step;
-    if (c->dp.rep || (c->p.openbc && c->p.internal_obstacles)) {
+    if (c->dp.rep) {
       // the repulsive scan reads the normals of compute_norm's aux set, which
-      // the fused fast path does not form, and the open faces read the
-      // post-moments state S, which the fast path keeps in registers: run
-      // the reference's stage order
+      // the fused fast path does not form: run the reference's stage order
       int rc = staged_step(c);
       if (rc) return rc;
       continue;
     }
+    const bool obc = c->sopen != nullptr;
     if (c->p.twocomponent) {
       // compute_norm + div_theta_n (one tiled pass), then the ghost-phi
-      // update on the solid nodes, then the fused step
+      // update on the solid nodes (and the open faces' wall copies), then
+      // the fused step
       StageArgs a = stage_args(c);
       a.gf_from_pre = 1;
       HIPCHK(c, launch_phase_prep(a, c->prec, c->tz_prep, c->stream));
       if (c->p.internal_obstacles) HIPCHK(c, launch_stage(LBF_STAGE_PHI_INT_BOUNDARY, a, c->prec, c->stream));
+      if (obc) HIPCHK(c, launch_stage(LBF_STAGE_OPEN_BC_PHI, a, c->prec, c->stream));
       if (c->lag) HIPCHK(c, launch_phi_lagrange(a, c->prec, c->partial, c->nred, c->lag, false, c->stream));
     }
     int rc = run_fused(c, true);
     if (rc) return rc;
+    if (obc) {
+      // the open faces correct the fused output (which already holds the
+      // near-wall closure) from the post-moments state the kernel set aside
+      StageArgs a = stage_args(c);
+      a.S = c->sopen;
+      HIPCHK(c, launch_stage(LBF_STAGE_OPEN_BC_H, a, c->prec, c->stream));
+    }
     c->cur ^= 1;
     c->ucur ^= 1;
   }

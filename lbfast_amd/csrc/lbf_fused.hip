@@ -274,7 +274,16 @@ __host__ __device__ inline int fused_tiles(const FusedArgs& a) {
 // One block: tile `tile` (x fastest, then y, then z chunks) of the domain `a`.
 // OB: the domain has internal obstacles (isfluid loads, solid nodes, the
 // near-wall closures); without them that code is compiled out.
-template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB>
+// A node next to the wall of a non-periodic axis (coordinate 1 or n-2): a
+// superset of the first fluid layers of the open faces.
+__device__ __forceinline__ bool open_layer(const DevParams& dp, int x, int y, int z) {
+  return (!dp.pbc[0] && (x == 1 || x == dp.nx - 2)) || (!dp.pbc[1] && (y == 1 || y == dp.ny - 2)) ||
+         (!dp.pbc[2] && (z == 1 || z == dp.nz - 2));
+}
+
+// OPEN: open faces (with OB and the inline moments stage only): the kernel
+// also hands over the post-moments state of the wall-adjacent nodes
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB, bool OPEN = false>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
   typedef Codec<PREC> C;
   typedef typename C::R R;
@@ -542,6 +551,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
         tFx = C::round(tFx); tFy = C::round(tFy); tFz = C::round(tFz);
         Fx = tFx * rinv; Fy = tFy * rinv; Fz = tFz * rinv;
         if (out_node && own_plane && !solid) {
+          if constexpr (OPEN) {
+            if (isf == -1 && open_layer(dp, xs, ys, zsrc)) {
+              // open faces: the post-moments state of a first-layer node,
+              // which the open-face correction reads after this launch (the
+              // stage sequence reads it from S)
+              store_mom_b<PREC>((T*)a.sopen + (size_t)zw * plM, fs, off.pm, off.m, m, dp, 0u);
+            }
+          }
           if constexpr (TWO) {
             if (dp.dens) {
               T* uo = (T*)a.uout + (size_t)zw * ny * 3 * fs;
@@ -747,9 +764,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
 }
 
 // One domain per launch: 1-D grid, XCD-aware tile order.
-template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB>
+template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY, bool OB, bool OPEN>
 __global__ void __launch_bounds__((TY + 3) * 64) fused_kernel(FusedArgs a) {
-  fused_body<LAT, HO, PREC, TWO, INLINE, TY, OB>(a, xcd_tile(blockIdx.x, fused_tiles<TY>(a)));
+  fused_body<LAT, HO, PREC, TWO, INLINE, TY, OB, OPEN>(a, xcd_tile(blockIdx.x, fused_tiles<TY>(a)));
 }
 
 // Many domains per launch: the grid covers every domain's tiles in table
@@ -772,8 +789,19 @@ template <int LAT, bool HO, int PREC, bool TWO, bool INLINE, int TY>
 static hipError_t launch_t(const FusedArgs& a, hipStream_t s) {
   dim3 grid(fused_tiles<TY>(a));
   dim3 block((TY + 3) * 64);
-  if (a.dp.obst) hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, true>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, false>), grid, block, 0, s, a);
+  // open faces reach the fast path with obstacles and 32/64-bit storage only
+  constexpr bool open_ok = INLINE && prec_base(PREC) <= PREC_MIXED;
+  if (a.dp.obst) {
+    if constexpr (open_ok) {
+      if (a.sopen) {
+        hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, true, true>), grid, block, 0, s, a);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, true, false>), grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((fused_kernel<LAT, HO, PREC, TWO, INLINE, TY, false, false>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -22,6 +22,10 @@ struct FusedArgs {
   // when seedp is set: a device counter that captured graphs advance)
   uint32_t seed;
   const uint32_t* seedp;
+  // open faces (inline moments stage only): 10-field set that receives the
+  // post-moments state of the nodes next to a non-periodic wall, read by
+  // the open-face correction after the launch; null without open faces
+  void* sopen;
 };
 
 // Stage-path buffers (the reference's per-kernel arguments), as field sets.

@@ -82,12 +82,12 @@ def test_create_rejects_bad_switches():
 
 def test_create_rejects_bad_open_boundaries():
     """Open faces exist only in the obstacle build (boundary_cds_module.f90:
-    49-57), run the stage sequence (no 16-bit storage) and take the types 0,
-    1 and 2; each violation is refused before the device is touched."""
+    49-57), take 32/64-bit storage and the types 0, 1 and 2; each violation
+    is refused before the device is touched."""
     L = hip.load()
     cases = [
         (dict(internal_obstacles=False), {}, b"INTERNAL_OBSTACLES"),
-        (dict(internal_obstacles=True, precision=3), {}, b"stage path"),
+        (dict(internal_obstacles=True, precision=3), {}, b"FP64, FP32 or MIXED"),
         (dict(internal_obstacles=True), {"type": 3}, b"openbc_type 3"),
         (dict(internal_obstacles=True), {"type": -1}, b"openbc_type -1"),
     ]

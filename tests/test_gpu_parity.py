@@ -177,8 +177,9 @@ def test_bcphiflux(name, prec, path):
 def test_open_faces(name, prec):
     """Open boundaries (bcs_mesoscopic_hfields / _phifields): pressure,
     velocity and local faces on all three axes, one and two components, and
-    BCPHIFLUX with the open faces' phase flux.  lbf_step runs the stage
-    order for them."""
+    BCPHIFLUX with the open faces' phase flux, on the fast path (the fused
+    kernel hands the correction the post-moments state of the wall-adjacent
+    nodes); test_open_faces_staged covers the stage sequence."""
     res, bad = run_pair(name, prec, (40, 24, 32), 10)
     assert not bad, bad
 
@@ -194,8 +195,9 @@ def test_open_faces_with_switches(name, prec):
 
 
 @pytest.mark.parametrize("name", ["duct27h_open", "chan27h_open_bcphi"])
-def test_open_faces_staged(name):
-    res, bad = run_pair(name, PREC_FP64, (33, 20, 18), 10, path="staged")
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_open_faces_staged(name, prec):
+    res, bad = run_pair(name, prec, (33, 20, 18), 10, path="staged")
     assert not bad, bad
 
 
