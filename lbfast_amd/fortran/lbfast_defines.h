@@ -28,11 +28,14 @@
 #if defined(LAPLACE) || defined(LAMBTEST)
 #define TWOCOMPONENT
 #define DENSRATIO
+#define PRINTPHI
+#define WRITEPRESS
 #endif
 #ifdef TWOPOISEUILLE
 #define INTERNAL_OBSTACLES
 #define TWOCOMPONENT
 #define DENSRATIO
+#define PRINTPHI
 #endif
 #ifdef POISEUILLE
 #define INTERNAL_OBSTACLES

@@ -21,7 +21,11 @@
   two-layer channel stays nearer its start than without it.
 * Open boundaries: a deck with pressure faces drives the flow from the high
   to the low pressure, and its frames equal a hip.Domain run bit for bit.
+* The starts of the Taylor-Green, Laplace, Poiseuille (with and without
+  bounce-back), two-fluid Poiseuille and Lamb builds equal the fields of the
+  reference's own initial-condition statements (tests/golden/ref_init.npz).
 """
+import json
 import os
 
 import numpy as np
@@ -110,10 +114,12 @@ def test_readinit_two_component_equals_domain(tmp_path):
     d.load(f)
     d.step(steps)
     phi = d.download(FIELD_PHI)
-    want_rho = _f32(1.0 * phi + (1.0 - phi) * 0.1)  # rho_r phi + (1 - phi) rho_b, as the frame writer
+    want_rho = _f32(phi)  # LAPLACE implies PRINTPHI (defines.h:38-43): the rho frame holds phi
     want_vel = np.stack([_f32(d.download(i)) for i in (2, 3, 4)], axis=-1)
+    want_p = _f32(d.download(1))  # and WRITEPRESS: P in out_press
     d.close()
     assert np.array_equal(fhost.read_frame(tmp_path, "rho", 1, n), want_rho)
+    assert np.array_equal(fhost.read_frame(tmp_path, "press", 1, n), want_p)
     assert np.array_equal(fhost.read_frame(tmp_path, "vel", 1, n, 3), want_vel)
 
 
@@ -201,9 +207,9 @@ def test_laplace_start_equals_python_droplet(tmp_path):
     d.close()
     got0 = fhost.read_frame(tmp_path, "rho", 0, n).astype(np.float64)
     got1 = fhost.read_frame(tmp_path, "rho", 1, n).astype(np.float64)
-    # frames hold rho_r phi + (1 - phi) rho_b in real(4)
-    assert np.abs(got0 - (want0 + (1 - want0) * 0.1)).max() < 1e-6
-    assert np.abs(got1 - (phi + (1 - phi) * 0.1)).max() < 1e-6
+    # frames hold phi (PRINTPHI) in real(4)
+    assert np.abs(got0 - want0).max() < 1e-6
+    assert np.abs(got1 - phi).max() < 1e-6
     gv = fhost.read_frame(tmp_path, "vel", 1, n, 3).astype(np.float64)
     assert np.abs(gv - vel).max() < 1e-7 * np.abs(vel).max()
     assert [r["step"] for r in rows] == [0, 2]
@@ -251,6 +257,51 @@ def test_poiseuille_bounce_back_channel(tmp_path):
     got = np.stack([_f32(d.download(i)) for i in (2, 3, 4)], axis=-1)
     d.close()
     assert np.array_equal(fhost.read_frame(tmp_path, "vel", 1, n, 3), got)
+
+
+_INIT = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_init.npz"))
+_INIT_META = json.loads(_INIT["meta"].tobytes().decode())
+
+
+@pytest.mark.parametrize("case,variant", [("tg", "tg27h_fp64"), ("laplace_2c", "laplace27h_fp64"),
+                                          ("pois", "pois19_fp64"), ("pois_bb", "pois19_bb_fp64"),
+                                          ("twopois", "twopois27h_fp64"), ("lamb", "lamb27h_fp32")])
+def test_host_start_matches_reference_statements(tmp_path, case, variant):
+    """Frame 0 of a host run (written after lbf_init_state) against the
+    fields the reference's initial-condition statements give on the same
+    grid (tests/golden/ref_init.npz; the Poiseuille noise term taken as 0,
+    which the host leaves out): the print observable (phi under PRINTPHI,
+    which LAPLACE, LAMBTEST and TWOPOISEUILLE imply, P otherwise), the
+    velocity and, where written, P.  Frames are real(4): agreement to one
+    float32 ulp.  Walls (solid nodes) are left out."""
+    m = _INIT_META[case]
+    n, q = tuple(m["n"]), m["params"]
+    want = _INIT[case + "/fields"]
+    two = "TWOCOMPONENT" in m["defines"]
+    keys = ("visc1", "fz", "uwall") + (("visc2", "rho_r", "rho_b", "sigma", "width", "radius", "center")
+                                        if two else ("width", "radius", "center"))
+    fl = {k: q[k] for k in keys if k in q}
+    fl.setdefault("visc1", 0.1)  # the viscosities do not enter the start
+    if two:
+        fl.setdefault("visc2", 0.1)
+        fl["tau_diff"] = 0.1
+    walls = case.startswith("pois") or case.startswith("twopois")
+    _run(variant, tmp_path, _sim(n, 2, stamp=2, stamp_term=2, pbc_x=0 if walls else 1), fl)
+    mask = np.ones((n[2], n[1], n[0]), dtype=bool)
+    if walls:
+        mask[:, :, 0] = mask[:, :, -1] = False
+
+    def agree(got, ref):
+        r = ref.astype(np.float32)
+        tol = np.spacing(np.abs(r)) + 1e-30
+        return bool(np.all(np.abs(got - r)[mask] <= tol[mask]))
+    obs = want[10] if two else want[0]
+    assert agree(fhost.read_frame(tmp_path, "rho", 0, n), obs), case
+    vel = fhost.read_frame(tmp_path, "vel", 0, n, 3)
+    for c in range(3):
+        assert agree(vel[..., c], want[1 + c]), (case, c)
+    if os.path.exists(tmp_path / "output" / "out_press_000000.raw"):
+        assert agree(fhost.read_frame(tmp_path, "press", 0, n), want[0]), case
 
 
 def test_open_faces_through_the_host(tmp_path):
