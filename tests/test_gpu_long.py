@@ -13,6 +13,10 @@
     from the phi(1-phi)-weighted interface positions along x through the
     centre (statistics.f90:160-268); sigma_eff agrees with sigma for the
     reference's density-matched deck and is steady at density ratio 100.
+* Open boundaries to steady state: a pressure-driven duct (pressure faces
+  in x, walls as velocity faces at rest) run 800 steps on the GPU equals the
+  FP64 oracle's run (FP64) and, in FP32, is the body-force duct of the
+  oracle's closed-form pin (developed profile within 5e-4 of the peak).
 """
 import numpy as np
 import pytest
@@ -230,3 +234,55 @@ def test_laplace_density_ratio_100():
     assert np.all(np.isfinite(v))
     assert abs(v[-1] - v[-2]) < 0.03
     assert 0.8 < v[-1] < 1.35
+
+
+_MOM = ("P", "u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
+
+
+def _open_duct(prec, n=(24, 8, 8), dP=1e-4):
+    p = make_params(n, lattice=27, highorder=True, internal_obstacles=True, pbc=(0, 0, 0), visc1=1.0 / 6.0,
+                    precision=prec)
+    p.openbc = 1
+    p.openbc_type[0], p.openbc_press[0] = 1, dP
+    p.openbc_type[1], p.openbc_press[1] = 1, 0.0
+    isf = hip.classify_isfluid(p)
+    shape = (n[2], n[1], n[0])
+    return p, {k: np.zeros(shape) for k in _MOM}, isf
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_open_duct_steady_state(prec):
+    n, steps = (24, 8, 8), 800
+    print(f"[long] open duct {n} prec={prec} {steps} steps", flush=True)
+    p, f, isf = _open_duct(prec, n)
+    d = hip.Domain(p)
+    d.load(f, isf)
+    d.step(steps)
+    got = np.stack([d.download(1 + k) for k in range(10)])
+    d.step(20)
+    later = np.stack([d.download(1 + k) for k in range(10)])
+    d.close()
+    p64, f, isf = _open_duct(PREC_FP64, n)
+    fl = isf != 0
+    assert np.abs(later - got)[:, fl].max() < (1e-9 if prec == PREC_FP64 else 1e-5) * np.abs(got[:, fl]).max()  # steady
+    o = oracle.Oracle(p64)
+    o.load(f, isf)
+    o.step(steps)
+    ref = np.stack([o.get_field(1 + k) for k in range(10)])
+    fluid = isf != 0  # solid nodes' outputs are never read (as in the parity tests)
+    if prec == PREC_FP64:
+        assert _rel(got[:, fluid], ref[:, fluid]) < 1e-11
+    # the developed profile is the body-force duct at the duct's own gradient
+    x = np.arange(1, n[0] + 1, dtype=float)
+    mid = slice(6, n[0] - 6)
+    c = np.polyfit(x[mid], got[0][4, 4, mid], 1)
+    pb = make_params((3, n[1], n[2]), lattice=27, highorder=True, internal_obstacles=True, pbc=(1, 0, 0),
+                     visc1=1.0 / 6.0, fx=-c[0] / 3.0, precision=PREC_FP64)
+    pb.openbc = 1
+    ob = oracle.Oracle(pb)
+    shape = (n[2], n[1], 3)
+    ob.load({k: np.zeros(shape) for k in _MOM}, oracle.classify_isfluid(pb, None))
+    ob.step(steps)
+    ub = ob.get_field(2)[1:-1, 1:-1, 1]  # the fluid cross-section (walls at y, z = 0 and n - 1)
+    u = got[1][1:-1, 1:-1, n[0] // 2]
+    assert np.abs(u - ub).max() < 5e-4 * ub.max(), np.abs(u - ub).max() / ub.max()
