@@ -258,18 +258,24 @@ template <> struct Codec<PREC_FP16S> {
   typedef __half T; typedef float R;
   static constexpr bool dev = true;
   __device__ static R dec(T v, float inv) { return __half2float(v) * inv; }
+  // a second moment from its stored deviation: v * inv + uu, one mixed-
+  // precision FMA (the half operand is widened inside the instruction)
+  __device__ static R decpi(T v, float inv, R uu) { return __builtin_fmaf(__half2float(v), inv, uu); }
   __device__ static T enc(R v, float s) { return __float2half_rn(v * s); }
-  // stochastic rounding of the scaled value x: 13 random bits (a uniform
-  // offset in [0, 1) ulp of the half significand, in the normal range where
-  // every stored scaled deviation sits) added to the magnitude, the bits
-  // below the half significand cleared, then an exact conversion: the value
-  // rounds away from zero with probability equal to the dropped fraction.
-  // The sign bit is untouched (the add cannot carry into it for finite x),
-  // and x = 0 stays 0.  Below the half normal range the conversion rounds
-  // the dithered value to nearest.  (gfx950's v_cvt_sr_f16_f32 does the same
-  // in one instruction but made the fused kernel 3 % slower.)
+  // stochastic rounding of the scaled value x: r holds 13 random bits (a
+  // uniform offset in [0, 1) ulp of the half significand, in the normal range
+  // where every stored scaled deviation sits), added to the magnitude, then a
+  // conversion toward zero, which drops the bits below the half significand:
+  // the value rounds away from zero with probability equal to the dropped
+  // fraction.  The sign bit is untouched (the add cannot carry into it for
+  // finite x), and x = 0 stays 0.  Below the half normal range the dither
+  // covers less than an ulp, so tiny values lean toward zero.  (gfx950's
+  // v_cvt_sr_f16_f32 rounds stochastically in one instruction but made the
+  // fused kernel 3 % slower.)
+  static constexpr uint32_t RBITS = 0x1FFFu;
   __device__ static T srx(float x, uint32_t r) {
-    return __float2half_rn(__uint_as_float((__float_as_uint(x) + (r & 0x1FFFu)) & ~0x1FFFu));
+    const auto h2 = __builtin_amdgcn_cvt_pkrtz(__uint_as_float(__float_as_uint(x) + r), 0.0f);
+    return __builtin_bit_cast(T, h2[0]);
   }
   __device__ static R round(R v) { return v; }
 };
@@ -277,11 +283,13 @@ template <> struct Codec<PREC_BF16S> {
   typedef __hip_bfloat16 T; typedef float R;
   static constexpr bool dev = true;
   __device__ static R dec(T v, float inv) { return __bfloat162float(v) * inv; }
+  __device__ static R decpi(T v, float inv, R uu) { return __builtin_fmaf(__bfloat162float(v), inv, uu); }
   __device__ static T enc(R v, float s) { return __float2bfloat16(v * s); }
-  // stochastic rounding of the scaled value x: 16 random bits added below the
-  // bfloat16 significand, then truncation
+  // stochastic rounding of the scaled value x: r holds 16 random bits, added
+  // below the bfloat16 significand, then truncation
+  static constexpr uint32_t RBITS = 0xFFFFu;
   __device__ static T srx(float x, uint32_t r) {
-    return __builtin_bit_cast(T, (unsigned short)((__float_as_uint(x) + (r & 0xFFFFu)) >> 16));
+    return __builtin_bit_cast(T, (unsigned short)((__float_as_uint(x) + r) >> 16));
   }
   __device__ static R round(R v) { return v; }
 };
@@ -340,9 +348,13 @@ __device__ __forceinline__ Mom<typename Codec<P>::R> load_mom(const typename Cod
   o.xy = C::dec(ldb(base + 7 * fs, boff), dp.invscale[7]);
   o.xz = C::dec(ldb(base + 8 * fs, boff), dp.invscale[8]);
   o.yz = C::dec(ldb(base + 9 * fs, boff), dp.invscale[9]);
-  if (C::dev) {
-    o.xx += o.u * o.u; o.yy += o.v * o.v; o.zz += o.w * o.w;
-    o.xy += o.u * o.v; o.xz += o.u * o.w; o.yz += o.v * o.w;
+  if constexpr (C::dev) {
+    o.xx = C::decpi(ldb(base + 4 * fs, boff), dp.invscale[4], o.u * o.u);
+    o.yy = C::decpi(ldb(base + 5 * fs, boff), dp.invscale[5], o.v * o.v);
+    o.zz = C::decpi(ldb(base + 6 * fs, boff), dp.invscale[6], o.w * o.w);
+    o.xy = C::decpi(ldb(base + 7 * fs, boff), dp.invscale[7], o.u * o.v);
+    o.xz = C::decpi(ldb(base + 8 * fs, boff), dp.invscale[8], o.u * o.w);
+    o.yz = C::decpi(ldb(base + 9 * fs, boff), dp.invscale[9], o.v * o.w);
   }
   return o;
 }
@@ -388,9 +400,13 @@ __device__ __forceinline__ Mom<typename Codec<P>::R> load_mom_b(const typename C
   o.xy = C::dec(fld(plane0, fs, 7, pb, boff), dp.invscale[7]);
   o.xz = C::dec(fld(plane0, fs, 8, pb, boff), dp.invscale[8]);
   o.yz = C::dec(fld(plane0, fs, 9, pb, boff), dp.invscale[9]);
-  if (C::dev) {
-    o.xx += o.u * o.u; o.yy += o.v * o.v; o.zz += o.w * o.w;
-    o.xy += o.u * o.v; o.xz += o.u * o.w; o.yz += o.v * o.w;
+  if constexpr (C::dev) {
+    o.xx = C::decpi(fld(plane0, fs, 4, pb, boff), dp.invscale[4], o.u * o.u);
+    o.yy = C::decpi(fld(plane0, fs, 5, pb, boff), dp.invscale[5], o.v * o.v);
+    o.zz = C::decpi(fld(plane0, fs, 6, pb, boff), dp.invscale[6], o.w * o.w);
+    o.xy = C::decpi(fld(plane0, fs, 7, pb, boff), dp.invscale[7], o.u * o.v);
+    o.xz = C::decpi(fld(plane0, fs, 8, pb, boff), dp.invscale[8], o.u * o.w);
+    o.yz = C::decpi(fld(plane0, fs, 9, pb, boff), dp.invscale[9], o.v * o.w);
   }
   return o;
 }
@@ -405,7 +421,8 @@ __device__ __forceinline__ void store_mom_b(typename Codec<P>::T* plane0, size_t
   if constexpr (C::dev) {
     // each field's random bits: the FP32 bits of the next field of the same
     // group (cyclic), XOR the seed
-    auto sr = [&](float x, float xn) { return C::srx(x, __float_as_uint(xn) ^ rnd); };
+    const uint32_t rm = rnd & C::RBITS;
+    auto sr = [&](float x, float xn) { return C::srx(x, (__float_as_uint(xn) & C::RBITS) ^ rm); };
     const float x0 = o.P * scl<P>(dp, 0), x1 = o.u * scl<P>(dp, 1), x2 = o.v * scl<P>(dp, 2), x3 = o.w * scl<P>(dp, 3);
     fst(plane0, fs, 0, pb, boff, sr(x0, x1));
     const T tu = sr(x1, x2), tv = sr(x2, x3), tw = sr(x3, x0);

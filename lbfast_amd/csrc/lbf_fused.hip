@@ -514,9 +514,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
       m.xx = C::dec(cur.m[4], iscl<PREC>(dp, 4)); m.yy = C::dec(cur.m[5], iscl<PREC>(dp, 5));
       m.zz = C::dec(cur.m[6], iscl<PREC>(dp, 6)); m.xy = C::dec(cur.m[7], iscl<PREC>(dp, 7));
       m.xz = C::dec(cur.m[8], iscl<PREC>(dp, 8)); m.yz = C::dec(cur.m[9], iscl<PREC>(dp, 9));
-      if (C::dev) {
-        m.xx += m.u * m.u; m.yy += m.v * m.v; m.zz += m.w * m.w;
-        m.xy += m.u * m.v; m.xz += m.u * m.w; m.yz += m.v * m.w;
+      if constexpr (C::dev) {
+        // second moments: stored deviation + u u (one mixed FMA each)
+        m.xx = C::decpi(cur.m[4], iscl<PREC>(dp, 4), m.u * m.u);
+        m.yy = C::decpi(cur.m[5], iscl<PREC>(dp, 5), m.v * m.v);
+        m.zz = C::decpi(cur.m[6], iscl<PREC>(dp, 6), m.w * m.w);
+        m.xy = C::decpi(cur.m[7], iscl<PREC>(dp, 7), m.u * m.v);
+        m.xz = C::decpi(cur.m[8], iscl<PREC>(dp, 8), m.u * m.w);
+        m.yz = C::decpi(cur.m[9], iscl<PREC>(dp, 9), m.v * m.w);
       }
       R phl = 0, Fx, Fy, Fz;
       const bool solid = obst && isf == 0;
@@ -564,11 +569,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int tile) {
               T* uo = (T*)a.uout + (size_t)zw * ny * 3 * fs;
               T eu = C::enc(m.u, scl<PREC>(dp, 1)), ev = C::enc(m.v, scl<PREC>(dp, 2)), ew = C::enc(m.w, scl<PREC>(dp, 3));
               if constexpr (C::dev) {
-                const uint32_t st = nseed ^ (uint32_t)zw * 0x27D4EB2Fu;
+                const uint32_t st = (nseed ^ (uint32_t)zw * 0x27D4EB2Fu) & C::RBITS;
                 const float xu = m.u * scl<PREC>(dp, 1), xv = m.v * scl<PREC>(dp, 2), xw_ = m.w * scl<PREC>(dp, 3);
-                eu = C::srx(xu, __float_as_uint(xv) ^ st);
-                ev = C::srx(xv, __float_as_uint(xw_) ^ st);
-                ew = C::srx(xw_, __float_as_uint(xu) ^ st);
+                eu = C::srx(xu, (__float_as_uint(xv) & C::RBITS) ^ st);
+                ev = C::srx(xv, (__float_as_uint(xw_) & C::RBITS) ^ st);
+                ew = C::srx(xw_, (__float_as_uint(xu) & C::RBITS) ^ st);
               }
               fst(uo, fs, 0, off.pu, off.u, eu);
               fst(uo, fs, 1, off.pu, off.u, ev);
