@@ -293,7 +293,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage legs")
-    ap.add_argument("--only", default="", help="run one leg: fp32 | fp16s | bf16s | 1c")
+    ap.add_argument("--only", default="", help="run one leg: fp32 | fp16s | bf16s | 1c | ensemble "
+                    "(the batched ensemble alone, --steps steps, for profiling)")
     ap.add_argument("--tile", default="", help="ty,tz override of the fused kernel tile")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -316,6 +317,14 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "ensemble":
+        ens = {}
+        for name, prec in (("fp32", PREC_FP32), ("fp16s", PREC_FP16S)):
+            print(f"[bench] batched ensemble {name}, {args.steps} steps", file=sys.stderr, flush=True)
+            ens[name], nodes_e = ensemble(prec, steps=args.steps)
+        print(json.dumps({"ensemble": {"domains": 96, "nodes": nodes_e, "steps": args.steps, **ens}}), flush=True)
+        return
+
     n = (args.n, args.n, args.n)
     nodes = n[0] * n[1] * n[2]
     tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
@@ -325,7 +334,7 @@ def main():
         legs = ["fp32", "1c"]
     bad = [x for x in legs if x not in precs and x != "1c"]
     if bad:
-        ap.error(f"--only: unknown leg {bad[0]} (fp32 | fp16s | bf16s | 1c)")
+        ap.error(f"--only: unknown leg {bad[0]} (fp32 | fp16s | bf16s | 1c | ensemble)")
     two_legs = [x for x in legs if x in precs]
     fields = None
     if two_legs or not args.no_cpu:
