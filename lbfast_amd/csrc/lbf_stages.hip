@@ -936,6 +936,42 @@ __device__ __forceinline__ bool open_face_node(const DevParams& dp, int f, int x
   return true;
 }
 
+// The open-face kernels run over the face layers only: blockIdx.y is the
+// face f (axis f >> 1, side f & 1) and the threads of a face cover its layer
+// at the given depth from the wall, the rectangle [1, n - 2] on the two other
+// axes (the nodes open_face_node accepts).  A node on several faces (depth
+// 1: where two open faces meet) is taken by the thread of its first face.
+__host__ __device__ inline size_t open_face_area(const DevParams& dp) {
+  const int n[3] = {dp.nx, dp.ny, dp.nz};
+  size_t m = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (dp.pbc[ax]) continue;
+    const int w1 = n[ax == 0 ? 1 : 0] - 2, w2 = n[ax == 2 ? 1 : 2] - 2;
+    if (w1 > 0 && w2 > 0 && (size_t)w1 * w2 > m) m = (size_t)w1 * w2;
+  }
+  return m;
+}
+
+__device__ __forceinline__ bool open_face_thread(const DevParams& dp, int depth, int& x, int& y, int& z) {
+  const int f = blockIdx.y, ax = f >> 1, side = f & 1;
+  if (dp.pbc[ax]) return false;
+  const int n[3] = {dp.nx, dp.ny, dp.nz};
+  const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
+  const int w1 = n[b1] - 2, w2 = n[b2] - 2;
+  if (w1 <= 0 || w2 <= 0) return false;
+  const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (t >= (size_t)w1 * w2) return false;
+  int c[3];
+  c[ax] = side ? n[ax] - 1 - depth : depth;
+  if (c[ax] < 0 || c[ax] >= n[ax]) return false;
+  c[b1] = 1 + (int)(t % (size_t)w1);
+  c[b2] = 1 + (int)(t / (size_t)w1);
+  x = c[0]; y = c[1]; z = c[2];
+  for (int g = 0; g < f; ++g)
+    if (open_face_node(dp, g, x, y, z, depth)) return false;
+  return true;
+}
+
 // Phase field (boundary_cds_module.f90:1353-1664): a wall node of an open
 // face takes phi of the node next to it inside the domain.  The faces do not
 // overlap (edges are skipped) and no node read is written.
@@ -944,7 +980,7 @@ __global__ void __launch_bounds__(BS) k_open_phi(StageArgs a) {
   typedef typename Aux<PREC>::T TA;
   const DevParams& dp = a.dp;
   int x, y, z;
-  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  if (!open_face_thread(dp, 0, x, y, z)) return;
   for (int f = 0; f < 6; ++f) {
     if (!open_face_node(dp, f, x, y, z, 0)) continue;
     const int ax = f >> 1, d = (f & 1) ? -1 : 1;
@@ -974,11 +1010,9 @@ __global__ void __launch_bounds__(BS) k_open_h(StageArgs a) {
   typedef typename Aux<PREC>::T TA;
   const DevParams& dp = a.dp;
   int x, y, z;
-  if (!node_of((size_t)blockIdx.x * BS + threadIdx.x, dp, x, y, z)) return;
+  if (!open_face_thread(dp, 1, x, y, z)) return;
   const int nx = dp.nx, ny = dp.ny, nz = dp.nz;
-  bool on = false;
-  for (int f = 0; f < 6; ++f) on = on || open_face_node(dp, f, x, y, z, 1);
-  if (!on || isf_at(a.isfh, x, y, z, nx, ny, nz) != -1) return;
+  if (isf_at(a.isfh, x, y, z, nx, ny, nz) != -1) return;
   const size_t n = nidx(x, y, z, nx, ny);
   const int sx = dp.sx;
   const size_t n10 = sbase(10, x, y, z, sx, ny);
@@ -1051,8 +1085,15 @@ static hipError_t stage_t(int stage, const StageArgs& a, hipStream_t s) {
     case 7: hipLaunchKernelGGL(k_lb_int_bnd<PREC>, grid, block, 0, s, a); break;
     case 8: hipLaunchKernelGGL(k_thinfilm_scan<PREC>, grid, block, 0, s, a); break;
     case 9: hipLaunchKernelGGL(k_repulsive_flux<PREC>, grid, block, 0, s, a); break;
-    case 11: hipLaunchKernelGGL(k_open_phi<PREC>, grid, block, 0, s, a); break;
-    case 12: hipLaunchKernelGGL(k_open_h<PREC>, grid, block, 0, s, a); break;
+    case 11:
+    case 12: {
+      const size_t m = open_face_area(a.dp);
+      if (m == 0) break;  // no open face has nodes
+      const dim3 fgrid((unsigned)((m + BS - 1) / BS), 6);
+      if (stage == 11) hipLaunchKernelGGL(k_open_phi<PREC>, fgrid, block, 0, s, a);
+      else hipLaunchKernelGGL(k_open_h<PREC>, fgrid, block, 0, s, a);
+      break;
+    }
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
